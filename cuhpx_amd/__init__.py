@@ -1,0 +1,26 @@
+"""cuhpx_amd: MI355X-native HEALPix spherical harmonic transforms and remaps.
+
+Public surface mirrors the reference package's hot path (cuhpx/__init__.py:16-19):
+SHTCUDA, iSHTCUDA, Grid, Regridding, ring2nest, nest2ring, nest2flat,
+flat2nest, ring2flat, flat2ring, flat2flat and sht_tools.  The device work is
+hand-written HIP for gfx950 in libhpxmi.so, reached through a C ABI
+(include/hpxmi.h).
+"""
+
+from . import sht_tools
+from .hpx_remap import flat2flat, flat2nest, flat2ring, nest2flat, nest2ring, ring2flat, ring2nest
+from .hpx_sht import SHTCUDA, ShtPlan, iSHTCUDA
+
+__all__ = [
+    "SHTCUDA",
+    "iSHTCUDA",
+    "ShtPlan",
+    "ring2nest",
+    "nest2ring",
+    "nest2flat",
+    "flat2nest",
+    "ring2flat",
+    "flat2ring",
+    "flat2flat",
+    "sht_tools",
+]

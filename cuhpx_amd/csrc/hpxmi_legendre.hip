@@ -1,0 +1,566 @@
+// Associated-Legendre step of the SHT on MFMA (gfx950).
+//
+// Replaces the reference's dense einsums (cuhpx/hpx_sht.py:557 forward,
+// :599 inverse, and the same contractions in the backward passes 573, 638),
+// which run one FP32 GEMM per m over all l and all rings, including the zero
+// triangle l < m, after a fresh pct*weights product on every call (546).
+//
+// MI355X design:
+//  * Equatorial folding.  P_lm(-z) = (-1)^(l+m) P_lm(z), so with the ring
+//    pair spectra S = F_t + F_t' and D = F_t - F_t' produced by the FFT stage,
+//      a[l, m] = sum_{k < npair} P_lm(z_k) (l+m even ? S_k : D_k)[m]
+//    and, inversely, Y_p[k] = sum_{l: l+m = p mod 2} P_lm(z_k) a[l, m].
+//    This halves both the flops and the table, and only l >= m is touched.
+//  * Table P[m][parity][i][k] (row i is l = m + parity + 2i) is generated on
+//    the device in FP64 by the three-term recurrence with a power-of-two
+//    exponent carried beside each value (no underflow to zero while the value
+//    can still matter), rounded once to FP32.
+//  * GEMM on MFMA: v_mfma_f32_16x16x4_f32 for the FP32 path, and
+//    v_mfma_f32_16x16x32_bf16 for the BF16 paths.  The 3-term split product
+//    hi*hi + hi*lo + lo*hi (x = hi + lo, both BF16, split from the FP32
+//    operands; the lo*lo term is below FP32 rounding) accumulates in FP32.
+//  * Forward work-group: one m, 64 rows of each parity, 128 columns
+//    (64 fields, re/im).  The table rows are staged once through LDS and
+//    shared by 4 waves; each wave streams its 32 columns of the spectra
+//    straight into MFMA B registers.  The output a[b][l][m] is m-minor, so the
+//    work list puts consecutive m of the same row block on the same XCD,
+//    whose L2 merges the partial-line writes.
+//  * Inverse work-group: one m, 64 ring pairs, 128 columns; the table tile is
+//    transposed through LDS.
+
+#include "hpxmi_sht_internal.h"
+#include "../../include/hpxmi.h"
+
+namespace hpxmi {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+constexpr int kRowsPerParity = 64;  // forward: rows i per parity per work-group
+constexpr int kKBlock = 64;         // inverse: ring pairs per work-group
+constexpr int kLdsPadF = 1;         // FP32 LDS row padding (floats)
+constexpr int kLdsPadH = 8;         // BF16 LDS row padding (bf16)
+
+__device__ __forceinline__ void split(float x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+// ---------------------------------------------------------------------------
+// Table generation (FP64, scaled recurrence)
+// ---------------------------------------------------------------------------
+
+// One thread per (m, k): P_mm by the sectoral product, then the three-term
+// recurrence in l (cuhpx/sht_tools.py:243-252), each value carried as
+// v * 2^(600 e); a value is stored when e == 0 and as 0 while e < 0
+// (|P| < 2^-600, far below FP32's range).
+__global__ __launch_bounds__(kThreads) void legendre_table_kernel(
+    float* __restrict__ P, const int64_t* __restrict__ p_off, const int* __restrict__ nl,
+    const double* __restrict__ ct, int npair, int kpad, int lmax) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  const int m = blockIdx.y;
+  if (k >= kpad) return;
+  const int nl0 = nl[2 * m], nl1 = nl[2 * m + 1];
+  const int pad0 = (nl0 + kRowPad - 1) / kRowPad * kRowPad;
+  const int pad1 = (nl1 + kRowPad - 1) / kRowPad * kRowPad;
+  float* P0 = P + p_off[m];
+  float* P1 = P0 + (int64_t)pad0 * kpad;
+  // padded rows and padded ring pairs are zero
+  for (int i = nl0; i < pad0; ++i) P0[(int64_t)i * kpad + k] = 0.f;
+  for (int i = nl1; i < pad1; ++i) P1[(int64_t)i * kpad + k] = 0.f;
+  if (k >= npair) {
+    for (int i = 0; i < nl0; ++i) P0[(int64_t)i * kpad + k] = 0.f;
+    for (int i = 0; i < nl1; ++i) P1[(int64_t)i * kpad + k] = 0.f;
+    return;
+  }
+  if (m >= lmax) return;
+  const double x = ct[k];
+  const double s = sqrt((1.0 - x) * (1.0 + x));
+  const double big = 0x1p600, tiny = 0x1p-600;
+  double pmm = 0.28209479177387814347;  // 1 / sqrt(4 pi)
+  int e = 0;
+  for (int i = 1; i <= m; ++i) {
+    pmm *= sqrt((2.0 * i + 1.0) / (2.0 * i)) * s;
+    if (pmm != 0.0 && fabs(pmm) < tiny) {
+      pmm *= big;
+      --e;
+    }
+  }
+  // l = m
+  P0[k] = e == 0 ? (float)pmm : 0.f;
+  if (m + 1 >= lmax) return;
+  // l = m + 1
+  double p2 = pmm;                              // P_{l-2}
+  double p1 = sqrt(2.0 * m + 3.0) * x * pmm;    // P_{l-1}
+  if (e < 0 && fabs(p1) > 1.0) {
+    p1 *= tiny;
+    p2 *= tiny;
+    ++e;
+  }
+  P1[k] = e == 0 ? (float)p1 : 0.f;
+  const double dm = m;
+  for (int l = m + 2; l < lmax; ++l) {
+    const double dl = l;
+    const double a = sqrt((2 * dl - 1) / (dl - dm) * (2 * dl + 1) / (dl + dm));
+    const double b =
+        sqrt((dl + dm - 1) / (dl - dm) * (2 * dl + 1) / (2 * dl - 3) * (dl - dm - 1) / (dl + dm));
+    const double v = x * a * p1 - b * p2;
+    p2 = p1;
+    p1 = v;
+    if (e < 0 && fabs(p1) > 1.0) {
+      p1 *= tiny;
+      p2 *= tiny;
+      ++e;
+    }
+    float* dst = ((l - m) & 1) ? P1 : P0;
+    dst[(int64_t)((l - m) >> 1) * kpad + k] = e == 0 ? (float)p1 : 0.f;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward: a[b][l][m] = sum_k P[m][p][i][k] X[m][p][k][col]
+// ---------------------------------------------------------------------------
+
+template <int PREC>
+struct FwdLds;
+
+template <>
+struct FwdLds<HPX_PREC_FP32> {
+  float a[2][kRowsPerParity][kKTile + kLdsPadF];
+};
+
+template <>
+struct FwdLds<HPX_PREC_BF16X3> {
+  __bf16 hi[2][kRowsPerParity][kKTile + kLdsPadH];
+  __bf16 lo[2][kRowsPerParity][kKTile + kLdsPadH];
+};
+
+template <>
+struct FwdLds<HPX_PREC_BF16> {
+  __bf16 hi[2][kRowsPerParity][kKTile + kLdsPadH];
+};
+
+template <int PREC>
+__global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
+    const float* __restrict__ X, float* __restrict__ alm, const float* __restrict__ P,
+    const int64_t* __restrict__ p_off, const int* __restrict__ nl,
+    const int2* __restrict__ items, int nitems, int npair, int kpad, int lmax, int mmax,
+    int64_t ncol, int64_t nbatch) {
+  __shared__ FwdLds<PREC> lds;
+  const int item = xcd_local_index(blockIdx.x, nitems);
+  const int m = items[item].x;
+  const int jb = items[item].y;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int col0 = blockIdx.y * kColBlock + wave * 32;
+  const bool wave_active = col0 < ncol;
+  const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
+  const int padp0 = (nlp[0] + kRowPad - 1) / kRowPad * kRowPad;
+  const int padp1 = (nlp[1] + kRowPad - 1) / kRowPad * kRowPad;
+  const int row0 = jb * kRowsPerParity;
+  const float* Pm = P + p_off[m];
+  const int64_t pstride = (int64_t)kpad * ncol;
+  const float* Xm = X + (int64_t)m * 2 * pstride;
+
+  floatx4 acc[2][4][2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) acc[p][rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  // staging assignment: 2 parities x 64 rows x 2 halves of 16 k
+  const int sp = threadIdx.x >> 7;
+  const int sr = (threadIdx.x >> 1) & 63;
+  const int sh = (threadIdx.x & 1) * 16;
+  const int srow = row0 + sr;
+  const bool srow_ok = srow < (sp ? padp1 : padp0);
+  const float* Psrc = Pm + (sp ? (int64_t)padp0 * kpad : 0) + (int64_t)srow * kpad + sh;
+
+  for (int kc = 0; kc < kpad; kc += kKTile) {
+    // ---- stage the table tile
+    {
+      float v[16];
+      if (srow_ok) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float4 t = *reinterpret_cast<const float4*>(Psrc + kc + 4 * q);
+          v[4 * q] = t.x;
+          v[4 * q + 1] = t.y;
+          v[4 * q + 2] = t.z;
+          v[4 * q + 3] = t.w;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = 0.f;
+      }
+      if constexpr (PREC == HPX_PREC_FP32) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) lds.a[sp][sr][sh + q] = v[q];
+      } else if constexpr (PREC == HPX_PREC_BF16X3) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) split(v[q], lds.hi[sp][sr][sh + q], lds.lo[sp][sr][sh + q]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) lds.hi[sp][sr][sh + q] = (__bf16)v[q];
+      }
+    }
+    // ---- this wave's B operand straight from the spectra
+    float bv[2][2][8];
+    if (wave_active) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          const int col = col0 + ct * 16 + (lane & 15);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            int kk;
+            if constexpr (PREC == HPX_PREC_FP32)
+              kk = kc + 4 * q + (lane >> 4);
+            else
+              kk = kc + 8 * (lane >> 4) + q;
+            bv[p][ct][q] = kk < npair ? Xm[p * pstride + (int64_t)kk * ncol + col] : 0.f;
+          }
+        }
+    }
+    __syncthreads();
+    if (wave_active) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          if (row0 + rt * 16 >= nlp[p]) continue;
+          const int ar = rt * 16 + (lane & 15);
+          if constexpr (PREC == HPX_PREC_FP32) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const float av = lds.a[p][ar][4 * q + (lane >> 4)];
+#pragma unroll
+              for (int ct = 0; ct < 2; ++ct)
+                acc[p][rt][ct] =
+                    __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
+            }
+          } else {
+            const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&lds.hi[p][ar][8 * (lane >> 4)]);
+            bf16x8 alo;
+            if constexpr (PREC == HPX_PREC_BF16X3)
+              alo = *reinterpret_cast<const bf16x8*>(&lds.lo[p][ar][8 * (lane >> 4)]);
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+              bf16x8 bhi, blo;
+#pragma unroll
+              for (int q = 0; q < 8; ++q) {
+                __bf16 h, l;
+                split(bv[p][ct][q], h, l);
+                bhi[q] = h;
+                blo[q] = l;
+              }
+              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[p][rt][ct], 0, 0, 0);
+              if constexpr (PREC == HPX_PREC_BF16X3) {
+                acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc[p][rt][ct], 0, 0, 0);
+                acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc[p][rt][ct], 0, 0, 0);
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (!wave_active) return;
+  // ---- write a[b][l][m]; lanes 2c and 2c+1 hold re and im of one field.
+  const bool odd = lane & 1;
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        float mine[4], other[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          mine[r] = acc[p][rt][ct][r];
+          other[r] = __shfl_xor(mine[r], 1);
+        }
+        const int col = col0 + ct * 16 + (lane & 15);
+        const int64_t b = col >> 1;
+        if (b >= nbatch) continue;
+#pragma unroll
+        for (int rr = 0; rr < 2; ++rr) {
+          const int r = odd ? rr + 2 : rr;
+          const int i = row0 + rt * 16 + 4 * ((lane >> 4) & 3) + r;
+          if (i >= nlp[p]) continue;
+          const int l = m + p + 2 * i;
+          const float2 v = odd ? make_float2(other[r], mine[r]) : make_float2(mine[r], other[r]);
+          *reinterpret_cast<float2*>(alm + 2 * ((b * lmax + l) * (int64_t)mmax + m)) = v;
+        }
+      }
+}
+
+// a[b][l][m] = 0 for m > l (the reference's dense output has zeros there).
+__global__ __launch_bounds__(kThreads) void zero_upper_kernel(float* __restrict__ alm, int lmax,
+                                                              int mmax, int64_t nbatch) {
+  const int64_t row = blockIdx.x;  // (b, l)
+  const int l = (int)(row % lmax);
+  const int m0 = l + 1;
+  if (m0 >= mmax) return;
+  float2* dst = reinterpret_cast<float2*>(alm) + row * mmax;
+  for (int m = m0 + threadIdx.x; m < mmax; m += kThreads) dst[m] = make_float2(0.f, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// Inverse: Y[m][p][k][col] = sum_i P[m][p][i][k] a[b][m+p+2i][m]
+// ---------------------------------------------------------------------------
+
+template <int PREC>
+struct InvLds;
+
+template <>
+struct InvLds<HPX_PREC_FP32> {
+  float a[2][kKBlock][kKTile + kLdsPadF];
+};
+
+template <>
+struct InvLds<HPX_PREC_BF16X3> {
+  __bf16 hi[2][kKBlock][kKTile + kLdsPadH];
+  __bf16 lo[2][kKBlock][kKTile + kLdsPadH];
+};
+
+template <>
+struct InvLds<HPX_PREC_BF16> {
+  __bf16 hi[2][kKBlock][kKTile + kLdsPadH];
+};
+
+template <int PREC>
+__global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
+    const float* __restrict__ alm, float* __restrict__ X, const float* __restrict__ P,
+    const int64_t* __restrict__ p_off, const int* __restrict__ nl, int nkb, int kpad, int lmax,
+    int mmax, int64_t ncol, int64_t nbatch) {
+  __shared__ InvLds<PREC> lds;
+  const int item = xcd_local_index(blockIdx.x, gridDim.x);
+  const int kb = item / mmax;
+  const int m = item - kb * mmax;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int col0 = blockIdx.y * kColBlock + wave * 32;
+  const bool wave_active = col0 < ncol;
+  const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
+  const int padp0 = (nlp[0] + kRowPad - 1) / kRowPad * kRowPad;
+  const int padp1 = (nlp[1] + kRowPad - 1) / kRowPad * kRowPad;
+  const int nmax = nlp[0] > nlp[1] ? nlp[0] : nlp[1];
+  const float* Pm = P + p_off[m];
+  const int64_t pstride = (int64_t)kpad * ncol;
+  const int k0 = kb * kKBlock;
+
+  floatx4 acc[2][4][2];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) acc[p][rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  // staging: 2 parities x 32 rows i x 4 quarters of 16 ring pairs
+  const int sp = threadIdx.x >> 7;
+  const int si = (threadIdx.x >> 2) & 31;
+  const int sq = (threadIdx.x & 3) * 16;
+  const int spad = sp ? padp1 : padp0;
+  const float* Pbase = Pm + (sp ? (int64_t)padp0 * kpad : 0);
+
+  for (int ic = 0; ic < nmax; ic += kKTile) {
+    {
+      float v[16];
+      const int irow = ic + si;
+      const int kk = k0 + sq;
+      if (irow < spad && kk < kpad) {
+        const float* src = Pbase + (int64_t)irow * kpad + kk;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
+          v[4 * q] = t.x;
+          v[4 * q + 1] = t.y;
+          v[4 * q + 2] = t.z;
+          v[4 * q + 3] = t.w;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = 0.f;
+      }
+      if constexpr (PREC == HPX_PREC_FP32) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) lds.a[sp][sq + q][si] = v[q];
+      } else if constexpr (PREC == HPX_PREC_BF16X3) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) split(v[q], lds.hi[sp][sq + q][si], lds.lo[sp][sq + q][si]);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) lds.hi[sp][sq + q][si] = (__bf16)v[q];
+      }
+    }
+    float bv[2][2][8];
+    if (wave_active) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          const int col = col0 + ct * 16 + (lane & 15);
+          const int64_t b = col >> 1;
+          const int reim = col & 1;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            int i;
+            if constexpr (PREC == HPX_PREC_FP32)
+              i = ic + 4 * q + (lane >> 4);
+            else
+              i = ic + 8 * (lane >> 4) + q;
+            float val = 0.f;
+            if (i < nlp[p] && b < nbatch) {
+              const int l = m + p + 2 * i;
+              val = alm[2 * ((b * lmax + l) * (int64_t)mmax + m) + reim];
+            }
+            bv[p][ct][q] = val;
+          }
+        }
+    }
+    __syncthreads();
+    if (wave_active) {
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        if (ic >= nlp[p]) continue;
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          const int ar = rt * 16 + (lane & 15);
+          if constexpr (PREC == HPX_PREC_FP32) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+              const float av = lds.a[p][ar][4 * q + (lane >> 4)];
+#pragma unroll
+              for (int ct = 0; ct < 2; ++ct)
+                acc[p][rt][ct] =
+                    __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
+            }
+          } else {
+            const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&lds.hi[p][ar][8 * (lane >> 4)]);
+            bf16x8 alo;
+            if constexpr (PREC == HPX_PREC_BF16X3)
+              alo = *reinterpret_cast<const bf16x8*>(&lds.lo[p][ar][8 * (lane >> 4)]);
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+              bf16x8 bhi, blo;
+#pragma unroll
+              for (int q = 0; q < 8; ++q) {
+                __bf16 h, l;
+                split(bv[p][ct][q], h, l);
+                bhi[q] = h;
+                blo[q] = l;
+              }
+              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[p][rt][ct], 0, 0, 0);
+              if constexpr (PREC == HPX_PREC_BF16X3) {
+                acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc[p][rt][ct], 0, 0, 0);
+                acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc[p][rt][ct], 0, 0, 0);
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (!wave_active) return;
+  float* Xm = X + (int64_t)m * 2 * pstride;
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        const int col = col0 + ct * 16 + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int k = k0 + rt * 16 + 4 * (lane >> 4) + r;
+          if (k < kpad) Xm[p * pstride + (int64_t)k * ncol + col] = acc[p][rt][ct][r];
+        }
+      }
+}
+
+}  // namespace
+
+int launch_legendre_table(ShtPlan& p, const double* d_ct, hipStream_t s) {
+  dim3 grid((p.kpad + kThreads - 1) / kThreads, p.mmax);
+  legendre_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
+                                                  p.lmax);
+  HPX_LAUNCH_CHECK();
+  return HPX_OK;
+}
+
+int launch_legendre_fwd(const ShtPlan& p, int precision, const float* X, float* alm,
+                        int64_t nbatch, hipStream_t s) {
+  if (nbatch <= 0) return HPX_OK;
+  const int64_t ncol = col_pad(nbatch);
+  const int64_t ycols = (ncol + kColBlock - 1) / kColBlock;
+  if (ycols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  if (p.n_fwd_items > 0) {
+    dim3 grid(p.n_fwd_items, (unsigned)ycols);
+    switch (precision) {
+      case HPX_PREC_FP32:
+        legendre_fwd_kernel<HPX_PREC_FP32><<<grid, kThreads, 0, s>>>(
+            X, alm, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items, p.n_fwd_items, p.npair, p.kpad,
+            p.lmax, p.mmax, ncol, nbatch);
+        break;
+      case HPX_PREC_BF16X3:
+        legendre_fwd_kernel<HPX_PREC_BF16X3><<<grid, kThreads, 0, s>>>(
+            X, alm, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items, p.n_fwd_items, p.npair, p.kpad,
+            p.lmax, p.mmax, ncol, nbatch);
+        break;
+      case HPX_PREC_BF16:
+        legendre_fwd_kernel<HPX_PREC_BF16><<<grid, kThreads, 0, s>>>(
+            X, alm, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items, p.n_fwd_items, p.npair, p.kpad,
+            p.lmax, p.mmax, ncol, nbatch);
+        break;
+      default:
+        HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
+    }
+    HPX_LAUNCH_CHECK();
+  }
+  if (p.mmax > 1) {
+    zero_upper_kernel<<<(unsigned)(nbatch * p.lmax), kThreads, 0, s>>>(alm, p.lmax, p.mmax, nbatch);
+    HPX_LAUNCH_CHECK();
+  }
+  return HPX_OK;
+}
+
+int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float* X,
+                        int64_t nbatch, hipStream_t s) {
+  if (nbatch <= 0) return HPX_OK;
+  const int64_t ncol = col_pad(nbatch);
+  const int64_t ycols = (ncol + kColBlock - 1) / kColBlock;
+  if (ycols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  const int nkb = (p.kpad + kKBlock - 1) / kKBlock;
+  dim3 grid((unsigned)(nkb * p.mmax), (unsigned)ycols);
+  switch (precision) {
+    case HPX_PREC_FP32:
+      legendre_inv_kernel<HPX_PREC_FP32><<<grid, kThreads, 0, s>>>(
+          alm, X, p.d_p, p.d_p_off, p.d_nl, nkb, p.kpad, p.lmax, p.mmax, ncol, nbatch);
+      break;
+    case HPX_PREC_BF16X3:
+      legendre_inv_kernel<HPX_PREC_BF16X3><<<grid, kThreads, 0, s>>>(
+          alm, X, p.d_p, p.d_p_off, p.d_nl, nkb, p.kpad, p.lmax, p.mmax, ncol, nbatch);
+      break;
+    case HPX_PREC_BF16:
+      legendre_inv_kernel<HPX_PREC_BF16><<<grid, kThreads, 0, s>>>(
+          alm, X, p.d_p, p.d_p_off, p.d_nl, nkb, p.kpad, p.lmax, p.mmax, ncol, nbatch);
+      break;
+    default:
+      HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
+  }
+  HPX_LAUNCH_CHECK();
+  return HPX_OK;
+}
+
+}  // namespace hpxmi

@@ -1,0 +1,354 @@
+// RING / NESTED / flat pixel remaps for gfx950.
+//
+// Every remap kind is a permutation of the 12*nside^2 pixels of a map that
+// is the same for every row of the batch.  The index arithmetic follows the
+// reference's kernels (src/data_remapping/hpx_remapping_cuda.cu):
+//   ring <-> (ix, iy, face)  161-282,  nest <-> (ix, iy, face)  285-347,
+//   flat <-> (ix, iy, face) with correct_xy_orient  33-134,
+// and the scatter semantics out[sigma(i)] = in[i] (137-146, 902-917).
+//
+// MI355X design: instead of materialising a pix array and scattering every
+// row through it (4 cudaMalloc'd index arrays, 4 launches, uncoalesced writes
+// in the reference), one launch walks the map in tiles.  A tile is a
+// TB x TB block of one base face (TB = min(nside, 64)), which is a contiguous
+// Morton range in NEST order, TB rows of TB in the flat layout, and 2*TB-1
+// contiguous ring segments in RING order.  Each work-group computes the
+// source and destination index of its tile's pixels once, in registers, and
+// then streams the batch rows through LDS: the source side is read in its
+// own natural order (coalesced), the destination side is written in its own
+// natural order (coalesced).
+
+#include "hpxmi_common.h"
+#include "../../include/hpxmi.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxTile = 64;                                  // TB
+constexpr int kPerThread = kMaxTile * kMaxTile / kThreads;    // 16
+
+enum Scheme { kNest = 0, kRing = 1, kFlat = 2 };
+
+struct Side {
+  int scheme;
+  int flip;  // flat only: swap ix <-> iy before rotating
+  int k;     // flat only: quarter turns
+};
+
+__device__ __forceinline__ int spread_bits(int v) {
+  int r = v & 0xffff;
+  r = (r ^ (r << 8)) & 0x00ff00ff;
+  r = (r ^ (r << 4)) & 0x0f0f0f0f;
+  r = (r ^ (r << 2)) & 0x33333333;
+  r = (r ^ (r << 1)) & 0x55555555;
+  return r;
+}
+
+__device__ __forceinline__ int compress_bits(int v) {
+  int r = v & 0x55555555;
+  r = (r ^ (r >> 1)) & 0x33333333;
+  r = (r ^ (r >> 2)) & 0x0f0f0f0f;
+  r = (r ^ (r >> 4)) & 0x00ff00ff;
+  r = (r ^ (r >> 8)) & 0x0000ffff;
+  return r;
+}
+
+// One quarter turn of correct_xy_orient (k == 1), masked to the face.
+__device__ __forceinline__ void rot1(int& x, int& y, int mask) {
+  int nx = (-y - 1) & mask;
+  int ny = x & mask;
+  x = nx;
+  y = ny;
+}
+
+// correct_xy_orient: optional swap, then k quarter turns.
+__device__ __forceinline__ void correct(int& x, int& y, int flip, int k, int mask) {
+  if (flip) {
+    int t = x;
+    x = y;
+    y = t;
+  }
+  for (int i = 0; i < k; ++i) rot1(x, y, mask);
+}
+
+// Inverse of correct(): (4 - k) quarter turns, then the optional swap.
+__device__ __forceinline__ void uncorrect(int& x, int& y, int flip, int k, int mask) {
+  for (int i = 0; i < ((4 - k) & 3); ++i) rot1(x, y, mask);
+  if (flip) {
+    int t = x;
+    x = y;
+    y = t;
+  }
+}
+
+// (ix, iy, face) -> RING index, as xyf2ring_kernel (hpx_remapping_cuda.cu:228-282).
+__device__ __forceinline__ int64_t xyf2ring(int ix, int iy, int face, int64_t nside) {
+  const int64_t nl4 = 4 * nside;
+  const int64_t npface = nside * nside;
+  const int64_t ncap = (npface - nside) << 1;
+  const int64_t npix = 12 * npface;
+  const int64_t jr = ((face >> 2) + 2) * nside - ix - iy - 1;
+  int64_t nr, n_before;
+  bool shifted;
+  if (jr < nside) {
+    shifted = true;
+    nr = 4 * jr;
+    n_before = 2 * jr * (jr - 1);
+  } else if (jr < 3 * nside) {
+    shifted = ((jr - nside) & 1) == 0;
+    nr = 4 * nside;
+    n_before = ncap + (jr - nside) * nr;
+  } else {
+    shifted = true;
+    const int64_t ring_nr = 4 * nside - jr;
+    nr = 4 * ring_nr;
+    n_before = npix - 2 * ring_nr * (ring_nr + 1);
+  }
+  nr >>= 2;
+  const int64_t kshift = shifted ? 0 : 1;
+  int64_t jp = ((face & 3) << 1) - ((face >> 2) & 1) + 1;
+  jp = (jp * nr + ix - iy + 1 + kshift) / 2;  // C division, as the reference
+  if (jp < 1) jp += nl4;
+  return n_before + jp - 1;
+}
+
+struct TileGeom {
+  int face;
+  int x0, y0;         // tile corner in face coordinates
+  int64_t nest_base;  // NEST index of the tile's first pixel
+  int tb, log_tb;
+};
+
+// Tile-local coordinates (lx, ly) and global index of the e-th pixel of the
+// tile in the natural order of `side`.
+__device__ __forceinline__ void element(const Side& side, const TileGeom& g, int e,
+                                        int64_t nside, int order, int fx0, int fy0,
+                                        int& lx, int& ly, int64_t& gidx) {
+  const int tb = g.tb;
+  const int mask = (int)nside - 1;
+  if (side.scheme == kNest) {
+    lx = compress_bits(e);
+    ly = compress_bits(e >> 1);
+    gidx = g.nest_base + e;
+  } else if (side.scheme == kFlat) {
+    // Row-major over the tile's image block in the flat layout.
+    const int r = e >> g.log_tb;
+    const int c = e & (tb - 1);
+    int x = fx0 + c;
+    int y = fy0 + r;
+    gidx = ((int64_t)g.face << (2 * order)) + ((int64_t)y << order) + x;
+    uncorrect(x, y, side.flip, side.k, mask);
+    lx = x - g.x0;
+    ly = y - g.y0;
+  } else {
+    // Ring segments s = lx + ly, each ordered by increasing lx (increasing
+    // position along the ring); first the lower triangle, then the upper.
+    const int tri = tb * (tb + 1) / 2;
+    if (e < tri) {
+      int s = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);
+      while ((s + 1) * (s + 2) / 2 <= e) ++s;
+      while (s * (s + 1) / 2 > e) --s;
+      const int pos = e - s * (s + 1) / 2;
+      lx = pos;
+      ly = s - pos;
+    } else {
+      const int ep = tb * tb - 1 - e;
+      int s = (int)((sqrtf(8.0f * ep + 1.0f) - 1.0f) * 0.5f);
+      while ((s + 1) * (s + 2) / 2 <= ep) ++s;
+      while (s * (s + 1) / 2 > ep) --s;
+      const int pos = ep - s * (s + 1) / 2;
+      lx = tb - 1 - pos;
+      ly = tb - 1 - (s - pos);
+    }
+    gidx = xyf2ring(g.x0 + lx, g.y0 + ly, g.face, nside);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void remap_tiles(
+    const T* __restrict__ in, T* __restrict__ out, int64_t nside, int order,
+    int tb, int log_tb, int tiles_per_face, int64_t nrows, int rows_per_chunk,
+    Side src, Side dst) {
+  __shared__ T lds[2][kMaxTile * kMaxTile];
+
+  const int ntiles = gridDim.x;
+  const int tile = xcd_local_index(blockIdx.x, ntiles);
+  TileGeom g;
+  g.face = tile / tiles_per_face;
+  const int b = tile - g.face * tiles_per_face;
+  g.tb = tb;
+  g.log_tb = log_tb;
+  g.x0 = compress_bits(b) << log_tb;
+  g.y0 = compress_bits(b >> 1) << log_tb;
+  g.nest_base = ((int64_t)g.face << (2 * order)) + ((int64_t)b << (2 * log_tb));
+  const int mask = (int)nside - 1;
+
+  // Corner of the tile's image block in each flat layout.
+  int sfx = 0, sfy = 0, dfx = 0, dfy = 0;
+  if (src.scheme == kFlat) {
+    int ax = g.x0, ay = g.y0, bx = g.x0 + tb - 1, by = g.y0 + tb - 1;
+    correct(ax, ay, src.flip, src.k, mask);
+    correct(bx, by, src.flip, src.k, mask);
+    sfx = min(ax, bx);
+    sfy = min(ay, by);
+  }
+  if (dst.scheme == kFlat) {
+    int ax = g.x0, ay = g.y0, bx = g.x0 + tb - 1, by = g.y0 + tb - 1;
+    correct(ax, ay, dst.flip, dst.k, mask);
+    correct(bx, by, dst.flip, dst.k, mask);
+    dfx = min(ax, bx);
+    dfy = min(ay, by);
+  }
+
+  const int npt = tb * tb;
+  int src_q[kPerThread], dst_q[kPerThread];
+  int src_i[kPerThread], dst_i[kPerThread];  // < 2^31 for nside <= 8192
+#pragma unroll
+  for (int i = 0; i < kPerThread; ++i) {
+    const int e = threadIdx.x + i * kThreads;
+    src_q[i] = -1;
+    dst_q[i] = -1;
+    src_i[i] = 0;
+    dst_i[i] = 0;
+    if (e < npt) {
+      int lx, ly;
+      int64_t gi;
+      element(src, g, e, nside, order, sfx, sfy, lx, ly, gi);
+      src_q[i] = spread_bits(lx) | (spread_bits(ly) << 1);
+      src_i[i] = (int)gi;
+      element(dst, g, e, nside, order, dfx, dfy, lx, ly, gi);
+      dst_q[i] = spread_bits(lx) | (spread_bits(ly) << 1);
+      dst_i[i] = (int)gi;
+    }
+  }
+
+  const int64_t npix = 12 * nside * nside;
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+  int64_t r1 = r0 + rows_per_chunk;
+  if (r1 > nrows) r1 = nrows;
+  int buf = 0;
+  for (int64_t r = r0; r < r1; ++r) {
+    const T* src_row = in + r * npix;
+    T* dst_row = out + r * npix;
+    T v[kPerThread];
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i)
+      if (src_q[i] >= 0) v[i] = src_row[src_i[i]];
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i)
+      if (src_q[i] >= 0) lds[buf][src_q[i]] = v[i];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kPerThread; ++i)
+      if (dst_q[i] >= 0) dst_row[dst_i[i]] = lds[buf][dst_q[i]];
+    buf ^= 1;
+  }
+}
+
+__global__ __launch_bounds__(256) void copy16(const uint4* __restrict__ in,
+                                              uint4* __restrict__ out, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = in[i];
+}
+
+__global__ void copy1(const uint8_t* __restrict__ in, uint8_t* __restrict__ out, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+// flip / k of a flat layout, as the reference wrappers derive them for the
+// NEST-side (origin S, counter-clockwise) partner: flip = cw,
+// k = (cw ? -o : o) & 3 (hpx_remapping_cuda.cu:415-418 with src (0, ccw)).
+Side flat_side(int origin, int cw) {
+  Side s;
+  s.scheme = kFlat;
+  s.flip = cw ? 1 : 0;
+  s.k = (cw ? -origin : origin) & 3;
+  return s;
+}
+
+template <typename T>
+int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Side dst,
+           hipStream_t stream) {
+  int order = 0;
+  while ((int64_t(1) << order) < nside) ++order;
+  const int tb = nside < kMaxTile ? (int)nside : kMaxTile;
+  int log_tb = 0;
+  while ((1 << log_tb) < tb) ++log_tb;
+  const int tiles_per_face = (int)((nside / tb) * (nside / tb));
+  const int ntiles = 12 * tiles_per_face;
+  // Enough work-groups to fill 256 CUs several times over.
+  int64_t chunks = (2048 + ntiles - 1) / ntiles;
+  if (chunks > nrows) chunks = nrows;
+  if (chunks < 1) chunks = 1;
+  const int rows_per_chunk = (int)((nrows + chunks - 1) / chunks);
+  chunks = (nrows + rows_per_chunk - 1) / rows_per_chunk;
+  dim3 grid(ntiles, (unsigned)chunks);
+  remap_tiles<T><<<grid, kThreads, 0, stream>>>(
+      static_cast<const T*>(in), static_cast<T*>(out), nside, order, tb, log_tb,
+      tiles_per_face, nrows, rows_per_chunk, src, dst);
+  HPX_LAUNCH_CHECK();
+  return HPX_OK;
+}
+
+}  // namespace
+
+extern "C" int hpx_remap(int kind, int64_t nside, int origin, int clockwise, int dst_origin,
+                         int dst_clockwise, const void* in, void* out, int64_t nrows,
+                         int elem_bytes, hipStream_t stream) {
+  if (nside < 1 || (nside & (nside - 1)) != 0)
+    HPX_FAIL(HPX_ERR_ARG, "nside must be a power of two (got %lld)", (long long)nside);
+  if (nside > 8192) HPX_FAIL(HPX_ERR_UNSUPPORTED, "nside > 8192 is not supported");
+  if (nrows < 0) HPX_FAIL(HPX_ERR_ARG, "nrows must be >= 0");
+  if (origin < 0 || origin > 3 || dst_origin < 0 || dst_origin > 3)
+    HPX_FAIL(HPX_ERR_ARG, "The origin must be one from S, E, N, W. Stop");
+  if (nrows == 0) return HPX_OK;
+  if (in == nullptr || out == nullptr) HPX_FAIL(HPX_ERR_ARG, "null data pointer");
+  if (in == out) HPX_FAIL(HPX_ERR_ARG, "in-place remap is not supported");
+
+  const Side nest{kNest, 0, 0};
+  const Side ring{kRing, 0, 0};
+  Side src, dst;
+  switch (kind) {
+    case HPX_RING2NEST: src = ring; dst = nest; break;
+    case HPX_NEST2RING: src = nest; dst = ring; break;
+    case HPX_NEST2FLAT: src = nest; dst = flat_side(origin, clockwise); break;
+    case HPX_FLAT2NEST: src = flat_side(origin, clockwise); dst = nest; break;
+    case HPX_RING2FLAT: src = ring; dst = flat_side(origin, clockwise); break;
+    case HPX_FLAT2RING: src = flat_side(origin, clockwise); dst = ring; break;
+    case HPX_FLAT2FLAT:
+      src = flat_side(origin, clockwise);
+      dst = flat_side(dst_origin, dst_clockwise);
+      break;
+    default: HPX_FAIL(HPX_ERR_ARG, "unknown remap kind %d", kind);
+  }
+  switch (elem_bytes) {
+    case 1: return launch<uint8_t>(in, out, nside, nrows, src, dst, stream);
+    case 2: return launch<uint16_t>(in, out, nside, nrows, src, dst, stream);
+    case 4: return launch<uint32_t>(in, out, nside, nrows, src, dst, stream);
+    case 8: return launch<uint64_t>(in, out, nside, nrows, src, dst, stream);
+    default: HPX_FAIL(HPX_ERR_ARG, "elem_bytes must be 1, 2, 4 or 8 (got %d)", elem_bytes);
+  }
+}
+
+extern "C" int hpx_copy(const void* in, void* out, int64_t nbytes, hipStream_t stream) {
+  if (nbytes < 0) HPX_FAIL(HPX_ERR_ARG, "nbytes must be >= 0");
+  if (nbytes == 0) return HPX_OK;
+  const bool aligned = ((reinterpret_cast<uintptr_t>(in) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+  const int64_t nvec = aligned ? nbytes / 16 : 0;
+  if (nvec > 0) {
+    int64_t blocks = (nvec + 255) / 256;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    copy16<<<(unsigned)blocks, 256, 0, stream>>>(static_cast<const uint4*>(in),
+                                                  static_cast<uint4*>(out), nvec);
+    HPX_LAUNCH_CHECK();
+  }
+  const int64_t tail = nbytes - nvec * 16;
+  if (tail > 0) {
+    copy1<<<(unsigned)((tail + 255) / 256), 256, 0, stream>>>(
+        static_cast<const uint8_t*>(in) + nvec * 16, static_cast<uint8_t*>(out) + nvec * 16, tail);
+    HPX_LAUNCH_CHECK();
+  }
+  return HPX_OK;
+}

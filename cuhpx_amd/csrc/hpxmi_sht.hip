@@ -1,0 +1,400 @@
+// SHT plans and the C-ABI entry points.
+//
+// A plan holds what the reference rebuilds per call or per module:
+//  * ring geometry of cuhpx/sht_tools.py:114-189, 319-332 (z, nphi, phi0,
+//    offsets) and the ring quadrature weights (sht_tools.py:47-59, passed in);
+//  * Bluestein tables for the rings whose length is not a power of two,
+//    computed in FP64 on the host: chirp exp(-i pi (j^2 mod 2N) / N) (the
+//    reduction mod 2N keeps the phase exact; the reference evaluates
+//    float(PI)*j*j/N in FP32, hpx_fft_cuda.cu:1090) and the kernel spectrum
+//    FFT_M(conj chirp) / M (the reference's y_pad FFT, hpx_fft.h:96-98);
+//  * per ring-pair phase factors exp(-i m phi0) (hpx_fft_cuda.cu:166-179);
+//  * the Legendre table, generated on the device (hpxmi_legendre.hip).
+
+#include "hpxmi_sht_internal.h"
+#include "../../include/hpxmi.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <string>
+
+namespace hpxmi {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+const char* last_error() { return g_error.c_str(); }
+
+namespace {
+
+using cd = std::complex<double>;
+
+bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+int ilog2(int64_t n) {
+  int r = 0;
+  while ((int64_t(1) << r) < n) ++r;
+  return r;
+}
+
+// In-place iterative radix-2 FFT, exp(-2 pi i jk / n), n a power of two.
+void fft64(std::vector<cd>& a) {
+  const size_t n = a.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (size_t len = 2; len <= n; len <<= 1) {
+    const size_t half = len / 2;
+    std::vector<cd> w(half);
+    for (size_t k = 0; k < half; ++k) {
+      const double ang = -2.0 * M_PI * (double)k / (double)len;
+      w[k] = cd(std::cos(ang), std::sin(ang));
+    }
+    for (size_t i = 0; i < n; i += len)
+      for (size_t k = 0; k < half; ++k) {
+        const cd u = a[i + k];
+        const cd v = a[i + k + half] * w[k];
+        a[i + k] = u + v;
+        a[i + k + half] = u - v;
+      }
+  }
+}
+
+float2 to_f2(cd z) { return make_float2((float)z.real(), (float)z.imag()); }
+
+template <typename T>
+int upload(T** dst, const std::vector<T>& src, int64_t& counter) {
+  if (src.empty()) {
+    *dst = nullptr;
+    return HPX_OK;
+  }
+  const size_t bytes = src.size() * sizeof(T);
+  HPX_HIP(hipMalloc(reinterpret_cast<void**>(dst), bytes));
+  HPX_HIP(hipMemcpy(*dst, src.data(), bytes, hipMemcpyHostToDevice));
+  counter += (int64_t)bytes;
+  return HPX_OK;
+}
+
+void free_plan(ShtPlan* p) {
+  if (p == nullptr) return;
+  hipFree(p->d_pairs);
+  hipFree(p->d_phase);
+  hipFree(p->d_chirp);
+  hipFree(p->d_bhat);
+  hipFree(p->d_twiddle);
+  hipFree(p->d_p);
+  hipFree(p->d_p_off);
+  hipFree(p->d_nl);
+  hipFree(p->d_fwd_items);
+  delete p;
+}
+
+// Builds a plan from ring geometry; cos_theta[t] is used for the Legendre
+// table of the north rings (t < npair).
+int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const double* phi0,
+               const double* weights, int lmax, int mmax, hipStream_t stream, ShtPlan** out) {
+  if (nring < 1) HPX_FAIL(HPX_ERR_ARG, "nring must be >= 1");
+  if (lmax < 1 || mmax < 1) HPX_FAIL(HPX_ERR_ARG, "lmax and mmax must be >= 1 (got %d, %d)", lmax, mmax);
+  if (mmax > 65535 || lmax > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "lmax/mmax too large");
+  for (int t = 0; t < nring; ++t) {
+    const int u = nring - 1 - t;
+    if (nphi[t] < 1) HPX_FAIL(HPX_ERR_ARG, "ring %d has no pixels", t);
+    if (nphi[t] != nphi[u]) HPX_FAIL(HPX_ERR_ARG, "grid is not mirror-symmetric (ring %d)", t);
+    if (std::fabs(phi0[t] - phi0[u]) > 1e-12) HPX_FAIL(HPX_ERR_ARG, "ring phases are not mirror-symmetric");
+    if (std::fabs(weights[t] - weights[u]) > 1e-12 * (std::fabs(weights[t]) + 1e-300))
+      HPX_FAIL(HPX_ERR_ARG, "ring weights are not mirror-symmetric");
+    if (std::fabs(cos_theta[t] + cos_theta[u]) > 1e-9) HPX_FAIL(HPX_ERR_ARG, "ring latitudes are not mirror-symmetric");
+  }
+
+  ShtPlan* p = new ShtPlan();
+  p->nring = nring;
+  p->npair = (nring + 1) / 2;
+  p->kpad = (p->npair + kKTile - 1) / kKTile * kKTile;
+  p->lmax = lmax;
+  p->mmax = mmax;
+
+  std::vector<int64_t> offset(nring + 1, 0);
+  for (int t = 0; t < nring; ++t) offset[t + 1] = offset[t] + nphi[t];
+  p->npix = offset[nring];
+
+  std::vector<float2> chirp, bhat;
+  p->pairs.resize(p->npair);
+  int max_M = 1;
+  for (int k = 0; k < p->npair; ++k) {
+    RingPair& r = p->pairs[k];
+    const int N = nphi[k];
+    r.nphi = N;
+    r.M = is_pow2(N) ? N : (int)(int64_t(1) << ilog2(2 * (int64_t)N - 1));
+    if (r.M > kFftLds) {
+      free_plan(p);
+      HPX_FAIL(HPX_ERR_UNSUPPORTED, "ring of %d pixels needs an FFT of %d > %d points", N, r.M, kFftLds);
+    }
+    r.log2M = ilog2(r.M);
+    r.single = (k == nring - 1 - k) ? 1 : 0;
+    r.off_n = offset[k];
+    r.off_s = offset[nring - 1 - k];
+    r.weight = (float)weights[k];
+    r.pad_ = 0.f;
+    r.chirp_off = 0;
+    r.bhat_off = 0;
+    max_M = std::max(max_M, r.M);
+    if (r.M != N) {
+      r.chirp_off = (int)chirp.size();
+      r.bhat_off = (int)bhat.size();
+      std::vector<cd> w(N);
+      for (int j = 0; j < N; ++j) {
+        const int64_t q = ((int64_t)j * j) % (2 * (int64_t)N);
+        const double ang = -M_PI * (double)q / (double)N;
+        w[j] = cd(std::cos(ang), std::sin(ang));
+        chirp.push_back(to_f2(w[j]));
+      }
+      std::vector<cd> b(r.M, cd(0.0, 0.0));
+      b[0] = std::conj(w[0]);
+      for (int j = 1; j < N; ++j) {
+        b[j] = std::conj(w[j]);
+        b[r.M - j] = std::conj(w[j]);
+      }
+      fft64(b);
+      for (int x = 0; x < r.M; ++x) bhat.push_back(to_f2(b[x] / (double)r.M));
+    }
+  }
+  p->max_M = max_M;
+  p->log2_max_M = ilog2(max_M);
+
+  std::vector<float2> tw(max_M);
+  for (int x = 0; x < max_M; ++x) {
+    const double ang = -2.0 * M_PI * (double)x / (double)max_M;
+    tw[x] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+  }
+
+  std::vector<float2> phase((size_t)p->npair * mmax);
+  for (int k = 0; k < p->npair; ++k)
+    for (int m = 0; m < mmax; ++m) {
+      const double ang = std::fmod((double)m * phi0[k], 2.0 * M_PI);
+      phase[(size_t)k * mmax + m] = make_float2((float)std::cos(ang), (float)-std::sin(ang));
+    }
+
+  // Legendre table layout
+  p->nl.resize(2 * (size_t)mmax);
+  p->p_off.resize(mmax + 1);
+  p->p_off[0] = 0;
+  std::vector<int2> items;
+  int max_blocks = 0;
+  for (int m = 0; m < mmax; ++m) {
+    for (int par = 0; par < 2; ++par) {
+      const int n = (lmax - m - par + 1) / 2;
+      p->nl[2 * m + par] = n > 0 ? n : 0;
+    }
+    const int pad0 = (p->nl[2 * m] + kRowPad - 1) / kRowPad * kRowPad;
+    const int pad1 = (p->nl[2 * m + 1] + kRowPad - 1) / kRowPad * kRowPad;
+    p->p_off[m + 1] = p->p_off[m] + (int64_t)(pad0 + pad1) * p->kpad;
+    const int nmax = std::max(p->nl[2 * m], p->nl[2 * m + 1]);
+    max_blocks = std::max(max_blocks, (nmax + 63) / 64);
+  }
+  // forward work list: row block outer, m inner (consecutive m share output
+  // cache lines and land on the same XCD)
+  for (int jb = 0; jb < max_blocks; ++jb)
+    for (int m = 0; m < mmax; ++m) {
+      const int nmax = std::max(p->nl[2 * m], p->nl[2 * m + 1]);
+      if (jb * 64 < nmax) items.push_back(make_int2(m, jb));
+    }
+  p->n_fwd_items = (int)items.size();
+
+  std::vector<double> ct(p->kpad, 0.0);
+  for (int k = 0; k < p->npair; ++k) ct[k] = cos_theta[k];
+
+  int64_t bytes = 0;
+  int rc;
+  if ((rc = upload(&p->d_pairs, p->pairs, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_phase, phase, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_chirp, chirp, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_bhat, bhat, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_twiddle, tw, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_p_off, p->p_off, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_nl, p->nl, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_fwd_items, items, bytes)) != HPX_OK) {
+    free_plan(p);
+    return rc;
+  }
+  // the FFT kernels read chirp / bhat through pointers even when unused
+  if (p->d_chirp == nullptr) {
+    std::vector<float2> one(1, make_float2(1.f, 0.f));
+    if ((rc = upload(&p->d_chirp, one, bytes)) != HPX_OK || (rc = upload(&p->d_bhat, one, bytes)) != HPX_OK) {
+      free_plan(p);
+      return rc;
+    }
+  }
+  const int64_t pfloats = p->p_off[mmax];
+  double* d_ct = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_p), std::max<int64_t>(pfloats, 1) * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_ct), ct.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(d_ct, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    hipFree(d_ct);
+    free_plan(p);
+    HPX_FAIL(HPX_ERR_HIP, "plan allocation failed: %s", hipGetErrorString(e));
+  }
+  bytes += pfloats * (int64_t)sizeof(float);
+  rc = launch_legendre_table(*p, d_ct, stream);
+  if (rc == HPX_OK) {
+    e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+      set_error(std::string("Legendre table generation failed: ") + hipGetErrorString(e));
+      rc = HPX_ERR_HIP;
+    }
+  }
+  hipFree(d_ct);
+  if (rc != HPX_OK) {
+    free_plan(p);
+    return rc;
+  }
+  p->table_bytes = bytes;
+  *out = p;
+  return HPX_OK;
+}
+
+int check_call(const ShtPlan* p, int precision, int64_t nbatch, int64_t ws_bytes) {
+  if (p == nullptr) HPX_FAIL(HPX_ERR_ARG, "null plan");
+  if (precision < 0 || precision > 2) HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
+  if (nbatch < 0) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 0");
+  const int64_t need = workspace_floats(*p, nbatch) * (int64_t)sizeof(float);
+  if (ws_bytes < need)
+    HPX_FAIL(HPX_ERR_ARG, "workspace of %lld bytes is smaller than the %lld bytes needed",
+             (long long)ws_bytes, (long long)need);
+  return HPX_OK;
+}
+
+}  // namespace
+}  // namespace hpxmi
+
+using namespace hpxmi;
+
+namespace {
+ShtPlan* P(const hpx_sht_plan* h) { return reinterpret_cast<ShtPlan*>(const_cast<hpx_sht_plan*>(h)); }
+hpx_sht_plan* H(ShtPlan* p) { return reinterpret_cast<hpx_sht_plan*>(p); }
+}  // namespace
+
+extern "C" const char* hpx_last_error(void) { return hpxmi::last_error(); }
+
+extern "C" int hpx_version(void) { return 100; }
+
+extern "C" int hpx_sht_plan_create_rings(int nring, const int32_t* nphi, const double* theta,
+                                         const double* phi0, const double* weights, int lmax,
+                                         int mmax, hipStream_t stream, hpx_sht_plan** plan) {
+  if (plan == nullptr || nphi == nullptr || theta == nullptr || phi0 == nullptr || weights == nullptr)
+    HPX_FAIL(HPX_ERR_ARG, "null argument");
+  std::vector<double> ct(nring > 0 ? nring : 0);
+  for (int t = 0; t < nring; ++t) ct[t] = std::cos(theta[t]);
+  ShtPlan* p = nullptr;
+  const int rc = build_plan(nring, nphi, ct.data(), phi0, weights, lmax, mmax, stream, &p);
+  if (rc != HPX_OK) return rc;
+  *plan = H(p);
+  return HPX_OK;
+}
+
+extern "C" int hpx_sht_plan_create_healpix(int nside, int lmax, int mmax, const double* ring_weights,
+                                           hipStream_t stream, hpx_sht_plan** plan) {
+  if (plan == nullptr || ring_weights == nullptr) HPX_FAIL(HPX_ERR_ARG, "null argument");
+  if (nside < 1) HPX_FAIL(HPX_ERR_ARG, "nside must be >= 1");
+  if (nside > 2048) HPX_FAIL(HPX_ERR_UNSUPPORTED, "nside > 2048 exceeds the FFT size limit");
+  const int nring = 4 * nside - 1;
+  std::vector<int32_t> nphi(nring);
+  std::vector<double> z(nring), phi0(nring);
+  const double ns = nside;
+  for (int t = 0; t < nring; ++t) {
+    // sht_tools.py:179-189 (nphi), 129-136 (z, before the flip), 161-176 (phi0)
+    if (t < nside - 1) {
+      nphi[t] = 4 * (t + 1);
+      z[t] = 1.0 - (double)(t + 1) * (t + 1) / (3.0 * ns * ns);
+    } else if (t <= 3 * nside - 1) {
+      nphi[t] = 4 * nside;
+      z[t] = 4.0 / 3.0 - 2.0 * (t + 1) / (3.0 * ns);
+    } else {
+      nphi[t] = 4 * (4 * nside - t - 1);
+      z[t] = (double)(4 * nside - 1 - t) * (4 * nside - 1 - t) / (3.0 * ns * ns) - 1.0;
+    }
+    if (t + 1 >= nside && t + 1 <= 3 * nside)
+      phi0[t] = M_PI / (2.0 * ns) * (0.5 * ((t - nside + 2) % 2));
+    else if (t + 1 > 3 * nside)
+      phi0[t] = M_PI / (2.0 * (4 * nside - t - 1)) * 0.5;
+    else
+      phi0[t] = M_PI / (2.0 * (t + 1)) * 0.5;
+  }
+  ShtPlan* p = nullptr;
+  const int rc = build_plan(nring, nphi.data(), z.data(), phi0.data(), ring_weights, lmax, mmax, stream, &p);
+  if (rc != HPX_OK) return rc;
+  *plan = H(p);
+  return HPX_OK;
+}
+
+extern "C" void hpx_sht_plan_destroy(hpx_sht_plan* plan) { free_plan(P(plan)); }
+
+extern "C" int64_t hpx_sht_plan_npix(const hpx_sht_plan* plan) { return plan ? P(plan)->npix : -1; }
+
+extern "C" int64_t hpx_sht_plan_table_bytes(const hpx_sht_plan* plan) {
+  return plan ? P(plan)->table_bytes : -1;
+}
+
+extern "C" int64_t hpx_sht_workspace_bytes(const hpx_sht_plan* plan, int64_t nbatch) {
+  if (plan == nullptr || nbatch < 0) return -1;
+  return workspace_floats(*P(plan), nbatch) * (int64_t)sizeof(float);
+}
+
+extern "C" int hpx_sht_forward(const hpx_sht_plan* plan, int precision, const float* f, float* alm,
+                               int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                               hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  int rc = check_call(pl, precision, nbatch, workspace_bytes);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  float* X = static_cast<float*>(workspace);
+  if ((rc = launch_analysis(*pl, f, X, nbatch, 1, 1.0f, 0, stream)) != HPX_OK) return rc;
+  return launch_legendre_fwd(*pl, precision, X, alm, nbatch, stream);
+}
+
+extern "C" int hpx_sht_inverse(const hpx_sht_plan* plan, int precision, const float* alm, float* f,
+                               int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                               hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  int rc = check_call(pl, precision, nbatch, workspace_bytes);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  float* X = static_cast<float*>(workspace);
+  if ((rc = launch_legendre_inv(*pl, precision, alm, X, nbatch, stream)) != HPX_OK) return rc;
+  return launch_synthesis(*pl, X, f, nbatch, 0, 1.0f, 0, stream);
+}
+
+extern "C" int hpx_sht_forward_adjoint(const hpx_sht_plan* plan, int precision, int mode,
+                                       const float* galm, float* gf, int64_t nbatch,
+                                       void* workspace, int64_t workspace_bytes,
+                                       hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  int rc = check_call(pl, precision, nbatch, workspace_bytes);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  if (mode != HPX_ADJ_EXACT && mode != HPX_ADJ_COMPAT) HPX_FAIL(HPX_ERR_ARG, "unknown adjoint mode %d", mode);
+  float* X = static_cast<float*>(workspace);
+  if ((rc = launch_legendre_inv(*pl, precision, galm, X, nbatch, stream)) != HPX_OK) return rc;
+  // exact: w_t * sum_{m<K} Re(H e^{i m phi});  compat: (w_t / 2) * irfft(H)
+  if (mode == HPX_ADJ_EXACT) return launch_synthesis(*pl, X, gf, nbatch, 1, 1.0f, 1, stream);
+  return launch_synthesis(*pl, X, gf, nbatch, 1, 0.5f, 0, stream);
+}
+
+extern "C" int hpx_sht_inverse_adjoint(const hpx_sht_plan* plan, int precision, int mode,
+                                       const float* gf, float* galm, int64_t nbatch,
+                                       void* workspace, int64_t workspace_bytes,
+                                       hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  int rc = check_call(pl, precision, nbatch, workspace_bytes);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  if (mode != HPX_ADJ_EXACT && mode != HPX_ADJ_COMPAT) HPX_FAIL(HPX_ERR_ARG, "unknown adjoint mode %d", mode);
+  float* X = static_cast<float*>(workspace);
+  // exact: c_m * rfft(g);  compat: 2 * rfft(g)
+  if (mode == HPX_ADJ_EXACT) rc = launch_analysis(*pl, gf, X, nbatch, 0, 1.0f, 1, stream);
+  else rc = launch_analysis(*pl, gf, X, nbatch, 0, 2.0f, 0, stream);
+  if (rc != HPX_OK) return rc;
+  return launch_legendre_fwd(*pl, precision, X, galm, nbatch, stream);
+}

@@ -1,0 +1,270 @@
+"""Spherical harmonic transforms on HEALPix (and other iso-latitude) grids.
+
+Drop-in for cuhpx/hpx_sht.py:515-735: ``SHTCUDA`` / ``iSHTCUDA`` take the
+same constructor arguments (nside, lmax, mmax, quad_weights, norm, csphase;
+defaults lmax = 4*nside-1, mmax = 2*nside+1, hpx_sht.py:654-655), map
+real [..., 12*nside^2] to complex [..., lmax, mmax] (a[l, m], zero for m > l)
+and back, raise ValueError on complex SHT input (687-688) and RuntimeError
+without a GPU (658-659), and are differentiable through autograd Functions
+with hand-written adjoints (515-641).  As in the reference, ``norm`` and
+``csphase`` are accepted and ignored (the functions are orthonormal, without
+the Condon-Shortley phase).
+
+Differences, all deliberate:
+  * the work runs on the caller's current stream (the reference pins the
+    stream current at construction, hpx_sht.py:656, 690);
+  * the input of the inverse transform is never modified;
+  * the backward passes are exact adjoints by default; ``adjoint="compat"``
+    reproduces the reference's formulas (0.5x / 2x on m = 0 and Nyquist);
+  * extra keyword ``precision`` selects the Legendre arithmetic: "fp32"
+    (default, FP32 MFMA), "bf16x3" (3-term split BF16 MFMA, FP32 accumulate)
+    or "bf16" (single-term BF16);
+  * float64 input is not supported (TypeError).
+
+All device work is in libhpxmi.so (ring FFTs, Legendre GEMMs, table
+generation); this module only allocates outputs and the workspace.
+"""
+
+import ctypes
+import threading
+
+import numpy as np
+import torch
+import torch.nn as nn
+
+from . import _lib
+from .sht_tools import W_helper, healpix_weights
+
+_plan_lock = threading.Lock()
+_plans = {}
+
+
+class ShtPlan:
+    """A device plan of libhpxmi.so for one grid, lmax, mmax and device."""
+
+    def __init__(self, handle, device, npix, nring, lmax, mmax):
+        self._handle = handle
+        self.device = device
+        self.npix = npix
+        self.nring = nring
+        self.lmax = lmax
+        self.mmax = mmax
+
+    @property
+    def handle(self):
+        return self._handle
+
+    @classmethod
+    def healpix(cls, nside, lmax, mmax, quad_weights="ring", device=None):
+        device = torch.device("cuda", torch.cuda.current_device() if device is None else device.index)
+        key = ("healpix", nside, lmax, mmax, quad_weights, device.index)
+        with _plan_lock:
+            plan = _plans.get(key)
+            if plan is None:
+                _, w = healpix_weights(nside, quad_weights)
+                w = np.ascontiguousarray(w, dtype=np.float64)
+                h = ctypes.c_void_p()
+                with torch.cuda.device(device):
+                    rc = _lib.lib().hpx_sht_plan_create_healpix(
+                        nside, lmax, mmax, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                        _lib.stream_ptr(device), ctypes.byref(h))
+                _lib.check(rc, "SHT plan")
+                plan = cls(h, device, 12 * nside * nside, 4 * nside - 1, lmax, mmax)
+                _plans[key] = plan
+        return plan
+
+    @classmethod
+    def rings(cls, nphi, theta, phi0, weights, lmax, mmax, device=None, key=None):
+        device = torch.device("cuda", torch.cuda.current_device() if device is None else device.index)
+        cache_key = None if key is None else ("rings", key, lmax, mmax, device.index)
+        with _plan_lock:
+            plan = _plans.get(cache_key) if cache_key is not None else None
+            if plan is None:
+                nphi = np.ascontiguousarray(nphi, dtype=np.int32)
+                theta = np.ascontiguousarray(theta, dtype=np.float64)
+                phi0 = np.ascontiguousarray(phi0, dtype=np.float64)
+                weights = np.ascontiguousarray(weights, dtype=np.float64)
+                dptr = ctypes.POINTER(ctypes.c_double)
+                h = ctypes.c_void_p()
+                with torch.cuda.device(device):
+                    rc = _lib.lib().hpx_sht_plan_create_rings(
+                        len(nphi), nphi.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                        theta.ctypes.data_as(dptr), phi0.ctypes.data_as(dptr),
+                        weights.ctypes.data_as(dptr), lmax, mmax, _lib.stream_ptr(device),
+                        ctypes.byref(h))
+                _lib.check(rc, "SHT plan")
+                plan = cls(h, device, int(nphi.sum()), len(nphi), lmax, mmax)
+                if cache_key is not None:
+                    _plans[cache_key] = plan
+        return plan
+
+    def __del__(self):
+        h = getattr(self, "_handle", None)
+        if h is not None and h.value:
+            try:
+                _lib.lib().hpx_sht_plan_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
+            self._handle = None
+
+    @property
+    def table_bytes(self):
+        return int(_lib.lib().hpx_sht_plan_table_bytes(self._handle))
+
+    def workspace(self, nbatch):
+        nbytes = int(_lib.lib().hpx_sht_workspace_bytes(self._handle, nbatch))
+        return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device), nbytes
+
+    def _run(self, fn, *args, nbatch):
+        ws, nbytes = self.workspace(nbatch)
+        with torch.cuda.device(self.device):
+            rc = fn(self._handle, *args, nbatch, _lib.ptr(ws), nbytes, _lib.stream_ptr(self.device))
+        return rc
+
+    def forward(self, f, precision):
+        """f [n, npix] float32 contiguous -> alm [n, lmax, mmax] complex64."""
+        n = f.shape[0]
+        out = torch.empty((n, self.lmax, self.mmax), dtype=torch.complex64, device=f.device)
+        rc = self._run(_lib.lib().hpx_sht_forward, _lib.PRECISIONS[precision], _lib.ptr(f),
+                       _lib.ptr(out), nbatch=n)
+        _lib.check(rc, "SHT forward")
+        return out
+
+    def inverse(self, alm, precision):
+        """alm [n, lmax, mmax] complex64 contiguous -> f [n, npix] float32."""
+        n = alm.shape[0]
+        out = torch.empty((n, self.npix), dtype=torch.float32, device=alm.device)
+        rc = self._run(_lib.lib().hpx_sht_inverse, _lib.PRECISIONS[precision], _lib.ptr(alm),
+                       _lib.ptr(out), nbatch=n)
+        _lib.check(rc, "SHT inverse")
+        return out
+
+    def forward_adjoint(self, galm, precision, mode):
+        n = galm.shape[0]
+        out = torch.empty((n, self.npix), dtype=torch.float32, device=galm.device)
+        rc = self._run(_lib.lib().hpx_sht_forward_adjoint, _lib.PRECISIONS[precision],
+                       _lib.ADJOINT_MODES[mode], _lib.ptr(galm), _lib.ptr(out), nbatch=n)
+        _lib.check(rc, "SHT forward adjoint")
+        return out
+
+    def inverse_adjoint(self, gf, precision, mode):
+        n = gf.shape[0]
+        out = torch.empty((n, self.lmax, self.mmax), dtype=torch.complex64, device=gf.device)
+        rc = self._run(_lib.lib().hpx_sht_inverse_adjoint, _lib.PRECISIONS[precision],
+                       _lib.ADJOINT_MODES[mode], _lib.ptr(gf), _lib.ptr(out), nbatch=n)
+        _lib.check(rc, "SHT inverse adjoint")
+        return out
+
+
+def _as_rows(x, width):
+    lead = x.shape[:-1]
+    return x.reshape(-1, width).contiguous(), lead
+
+
+def _as_alm_rows(a, lmax, mmax):
+    lead = a.shape[:-2]
+    return a.reshape(-1, lmax, mmax).contiguous(), lead
+
+
+class SHTFunction(torch.autograd.Function):
+    """Forward SHT with a hand-written adjoint (hpx_sht.py:515-583)."""
+
+    @staticmethod
+    def forward(ctx, x, plan, precision, adjoint):
+        ctx.plan, ctx.precision, ctx.adjoint = plan, precision, adjoint
+        rows, lead = _as_rows(x, plan.npix)
+        out = plan.forward(rows, precision)
+        return out.reshape(*lead, plan.lmax, plan.mmax)
+
+    @staticmethod
+    def backward(ctx, grad):
+        plan = ctx.plan
+        rows, lead = _as_alm_rows(grad.to(torch.complex64), plan.lmax, plan.mmax)
+        gx = plan.forward_adjoint(rows, ctx.precision, ctx.adjoint)
+        return gx.reshape(*lead, plan.npix), None, None, None
+
+
+class iSHTFunction(torch.autograd.Function):
+    """Inverse SHT with a hand-written adjoint (hpx_sht.py:586-641)."""
+
+    @staticmethod
+    def forward(ctx, a, plan, precision, adjoint):
+        ctx.plan, ctx.precision, ctx.adjoint = plan, precision, adjoint
+        rows, lead = _as_alm_rows(a, plan.lmax, plan.mmax)
+        out = plan.inverse(rows, precision)
+        return out.reshape(*lead, plan.npix)
+
+    @staticmethod
+    def backward(ctx, grad):
+        plan = ctx.plan
+        rows, lead = _as_rows(grad.to(torch.float32), plan.npix)
+        ga = plan.inverse_adjoint(rows, ctx.precision, ctx.adjoint)
+        return ga.reshape(*lead, plan.lmax, plan.mmax), None, None, None
+
+
+def _check_opts(precision, adjoint):
+    if precision not in _lib.PRECISIONS:
+        raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
+    if adjoint not in _lib.ADJOINT_MODES:
+        raise ValueError(f"adjoint must be one of {sorted(_lib.ADJOINT_MODES)}")
+
+
+class _HealpixSHTBase(nn.Module):
+    def __init__(self, nside, lmax=None, mmax=None, quad_weights="ring", norm="ortho", csphase=True,
+                 precision="fp32", adjoint="exact"):
+        super().__init__()
+        self.nside = nside
+        self.norm = norm
+        self.csphase = csphase
+        self.nlat = 4 * nside - 1
+        self.nlon = 4 * nside
+        self.quad_weights = quad_weights
+        self.lmax = lmax or self.nlat
+        self.mmax = mmax or (self.nlon // 2 + 1)
+        _check_opts(precision, adjoint)
+        self.precision = precision
+        self.adjoint = adjoint
+        if not torch.cuda.is_available():
+            raise RuntimeError("CUDA device is not available. This class requires a GPU.")
+        self.device = torch.device("cuda")
+        _, w = healpix_weights(nside, quad_weights)
+        self.register_buffer("weights", torch.from_numpy(w).to(torch.float32).to(self.device),
+                             persistent=False)
+        self.register_buffer("W", torch.from_numpy(W_helper(w, nside)).to(self.device),
+                             persistent=False)
+        self._plan = ShtPlan.healpix(nside, self.lmax, self.mmax, quad_weights)
+
+    def plan_for(self, device):
+        if device.type != "cuda":
+            raise RuntimeError("SHTCUDA / iSHTCUDA need a GPU tensor")
+        if self._plan.device.index != device.index:
+            self._plan = ShtPlan.healpix(self.nside, self.lmax, self.mmax, self.quad_weights, device)
+        return self._plan
+
+
+class SHTCUDA(_HealpixSHTBase):
+    """Forward SHT: real [..., 12 nside^2] (RING) -> complex [..., lmax, mmax]."""
+
+    def forward(self, x):
+        if torch.is_complex(x):
+            raise ValueError("Input tensor must be real.")
+        if x.dtype != torch.float32:
+            raise TypeError(f"SHTCUDA computes in float32; got {x.dtype}")
+        plan = self.plan_for(x.device)
+        if x.shape[-1] != plan.npix:
+            raise RuntimeError(f"last dimension must be {plan.npix}, got {x.shape[-1]}")
+        return SHTFunction.apply(x, plan, self.precision, self.adjoint)
+
+
+class iSHTCUDA(_HealpixSHTBase):
+    """Inverse SHT: complex [..., lmax, mmax] -> real [..., 12 nside^2] (RING)."""
+
+    def forward(self, x):
+        if not torch.is_complex(x):
+            raise ValueError("Input tensor must be complex.")
+        if x.dtype != torch.complex64:
+            raise TypeError(f"iSHTCUDA computes in complex64; got {x.dtype}")
+        plan = self.plan_for(x.device)
+        if tuple(x.shape[-2:]) != (plan.lmax, plan.mmax):
+            raise RuntimeError(f"last dimensions must be ({plan.lmax}, {plan.mmax}), got {tuple(x.shape[-2:])}")
+        return iSHTFunction.apply(x, plan, self.precision, self.adjoint)

@@ -1,0 +1,170 @@
+/* hpxmi: MI355X-native HEALPix spherical-harmonic transform and pixel
+ * remaps, exported as a plain C ABI.
+ *
+ * All device pointers are HIP device memory owned by the caller; outputs are
+ * allocated by the caller.  Every call is asynchronous on the given stream
+ * and returns HPX_OK or an error code; hpx_last_error() then describes the
+ * failure (thread-local).  No call allocates device memory except plan
+ * creation.
+ *
+ * Layouts (row-major, last index fastest):
+ *   map  f    float32 [nbatch, npix]                 (RING order for HEALPix)
+ *   alm  a    complex64 [nbatch, lmax, mmax]         (interleaved re, im;
+ *                                                      a[l, m] = 0 for m > l)
+ */
+#ifndef HPXMI_H
+#define HPXMI_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPX_OK 0
+#define HPX_ERR_ARG 1          /* invalid argument (size, nside, origin, ...) */
+#define HPX_ERR_HIP 2          /* HIP runtime error */
+#define HPX_ERR_UNSUPPORTED 3  /* valid request outside this build's limits */
+
+/* Description of the last error on the calling thread. */
+const char* hpx_last_error(void);
+
+/* Library ABI version (major * 100 + minor). */
+int hpx_version(void);
+
+/* ------------------------------------------------------------------ */
+/* Pixel remaps                                                        */
+/* ------------------------------------------------------------------ */
+
+/* Remap kinds.  Each replaces one pybind11 export of the reference's
+ * cuhpx_remap module (src/data_remapping/hpx_remapping.cpp:452-478); the
+ * 1-D and _batch forms are the same call with nrows = 1 or the product of
+ * the leading dimensions:
+ *   HPX_RING2NEST  ring2nest / ring2nest_batch   (hpx_remapping.cpp:21-32, 186-228)
+ *   HPX_NEST2RING  nest2ring / nest2ring_batch   (hpx_remapping.cpp:35-46)
+ *   HPX_NEST2FLAT  nest2xy / nest2xy_batch       (hpx_remapping.cpp:48-81)
+ *   HPX_FLAT2NEST  xy2nest / xy2nest_batch       (hpx_remapping.cpp:83-116)
+ *   HPX_RING2FLAT  ring2xy / ring2xy_batch       (hpx_remapping.cpp:118-150)
+ *   HPX_FLAT2RING  xy2ring / xy2ring_batch       (hpx_remapping.cpp:152-184)
+ *   HPX_FLAT2FLAT  xy2xy / xy2xy_batch           (hpx_remapping.cpp:406-448)
+ */
+enum {
+  HPX_RING2NEST = 0,
+  HPX_NEST2RING = 1,
+  HPX_NEST2FLAT = 2,
+  HPX_FLAT2NEST = 3,
+  HPX_RING2FLAT = 4,
+  HPX_FLAT2RING = 5,
+  HPX_FLAT2FLAT = 6
+};
+
+/* Origin codes of the flat (XY) layout, as hpx_remapping.cpp:65-75. */
+enum { HPX_ORIGIN_S = 0, HPX_ORIGIN_E = 1, HPX_ORIGIN_N = 2, HPX_ORIGIN_W = 3 };
+
+/* out[row, sigma(i)] = in[row, i] for every row (a pure permutation of
+ * elem_bytes-sized elements, bit-exact for any dtype).  nside must be a
+ * power of two; each row holds 12*nside^2 elements; in and out must not
+ * overlap.  For the one-sided flat kinds (origin, clockwise) describe the
+ * flat layout and the dst_* arguments are ignored; for HPX_FLAT2FLAT they
+ * describe the source layout and dst_* the destination layout.
+ * elem_bytes is 1, 2, 4 or 8. */
+int hpx_remap(int kind, int64_t nside, int origin, int clockwise,
+              int dst_origin, int dst_clockwise, const void* in, void* out,
+              int64_t nrows, int elem_bytes, hipStream_t stream);
+
+/* Plain device-to-device copy kernel (16-byte vector loads/stores); the
+ * bandwidth yardstick that remap and FFT traffic are scaled by.  No
+ * counterpart in the reference. */
+int hpx_copy(const void* in, void* out, int64_t nbytes, hipStream_t stream);
+
+/* ------------------------------------------------------------------ */
+/* Spherical harmonic transform                                        */
+/* ------------------------------------------------------------------ */
+
+typedef struct hpx_sht_plan hpx_sht_plan;
+
+/* Legendre-step arithmetic. */
+enum {
+  HPX_PREC_FP32 = 0,    /* FP32 MFMA, FP32 operands */
+  HPX_PREC_BF16X3 = 1,  /* BF16 MFMA, 3-term split (hi*hi + hi*lo + lo*hi), FP32 accumulate */
+  HPX_PREC_BF16 = 2     /* BF16 MFMA, single term */
+};
+
+/* Adjoint variants. */
+enum {
+  HPX_ADJ_EXACT = 0,   /* exact adjoint of the forward operator */
+  HPX_ADJ_COMPAT = 1   /* the reference's hand-written backward formulas
+                          (hpx_sht.py:564-583, 613-641) */
+};
+
+/* Plan for a HEALPix grid: the state SHTCUDA / iSHTCUDA build in their
+ * constructors (cuhpx/hpx_sht.py:645-683, 696-730): ring geometry, ring
+ * quadrature weights (ring_weights[t], t = 0..4*nside-2, FP64, as
+ * sht_tools.healpix_weights returns them), and the normalised Legendre table,
+ * which is generated on the device in FP64 by three-term recurrence with
+ * underflow scaling (replacing sht_tools.py:232-264).  Replaces the plan
+ * state of HealpixFFT / HealpixIFFT (src/harmonic_transform/hpx_fft.h:69-255).
+ * Synchronises the stream before returning. */
+int hpx_sht_plan_create_healpix(int nside, int lmax, int mmax,
+                                const double* ring_weights,
+                                hipStream_t stream, hpx_sht_plan** plan);
+
+/* Plan for a general iso-latitude grid that is mirror-symmetric about the
+ * equator (e.g. an equiangular lat/lon grid for Regridding, hpx_regrid.py:68-86):
+ * ring t has nphi[t] pixels at longitudes phi0[t] + 2*pi*j/nphi[t] and
+ * colatitude theta[t] (north to south); pixels are stored ring after ring.
+ * No counterpart in the reference (which delegates lat/lon grids to
+ * torch_harmonics). */
+int hpx_sht_plan_create_rings(int nring, const int32_t* nphi,
+                              const double* theta, const double* phi0,
+                              const double* weights, int lmax, int mmax,
+                              hipStream_t stream, hpx_sht_plan** plan);
+
+void hpx_sht_plan_destroy(hpx_sht_plan* plan);
+
+/* Pixels per map and device bytes of the plan's tables. */
+int64_t hpx_sht_plan_npix(const hpx_sht_plan* plan);
+int64_t hpx_sht_plan_table_bytes(const hpx_sht_plan* plan);
+
+/* Device workspace (bytes) that forward / inverse / adjoint calls on nbatch
+ * maps need; the caller allocates it (256-byte aligned). */
+int64_t hpx_sht_workspace_bytes(const hpx_sht_plan* plan, int64_t nbatch);
+
+/* Forward SHT, f[nbatch, npix] -> alm[nbatch, lmax, mmax] (complex64).
+ * Replaces SHTFunction.forward (hpx_sht.py:517-562): cuhpx_fft
+ * healpix_rfft_batch / healpix_rfft_class (hpx_fft.cpp:21-84, 152-193)
+ * followed by the einsum '...kmn,mlk->...lmn' (hpx_sht.py:557). */
+int hpx_sht_forward(const hpx_sht_plan* plan, int precision, const float* f,
+                    float* alm, int64_t nbatch, void* workspace,
+                    int64_t workspace_bytes, hipStream_t stream);
+
+/* Inverse SHT, alm[nbatch, lmax, mmax] -> f[nbatch, npix].
+ * Replaces iSHTFunction.forward (hpx_sht.py:588-611): the einsum
+ * '...lmn,mlk->...kmn' (hpx_sht.py:599) followed by cuhpx_fft
+ * healpix_irfft_batch / healpix_irfft_class (hpx_fft.cpp:86-149, 195-238).
+ * The input is not modified (the reference's irfft mutates its input). */
+int hpx_sht_inverse(const hpx_sht_plan* plan, int precision, const float* alm,
+                    float* f, int64_t nbatch, void* workspace,
+                    int64_t workspace_bytes, hipStream_t stream);
+
+/* Gradient of the forward SHT: galm[nbatch, lmax, mmax] -> gf[nbatch, npix].
+ * Replaces SHTFunction.backward (hpx_sht.py:564-583). */
+int hpx_sht_forward_adjoint(const hpx_sht_plan* plan, int precision, int mode,
+                            const float* galm, float* gf, int64_t nbatch,
+                            void* workspace, int64_t workspace_bytes,
+                            hipStream_t stream);
+
+/* Gradient of the inverse SHT: gf[nbatch, npix] -> galm[nbatch, lmax, mmax].
+ * Replaces iSHTFunction.backward (hpx_sht.py:613-641). */
+int hpx_sht_inverse_adjoint(const hpx_sht_plan* plan, int precision, int mode,
+                            const float* gf, float* galm, int64_t nbatch,
+                            void* workspace, int64_t workspace_bytes,
+                            hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPXMI_H */

@@ -1,0 +1,216 @@
+"""SHT kernels vs the FP64 oracle and the reference CPU torch path.
+
+Tolerances (relative L2 against the FP64 oracle of the same discrete
+operator): FP32 path 1e-5, split-BF16 (3 terms) 2e-5, single-term BF16 1e-2.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import cuhpx_amd
+from cuhpx_amd.hpx_sht import ShtPlan
+from oracle import sht as S
+from oracle.cpu_path import CPUTorchSHT
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": 1e-5, "bf16x3": 2e-5, "bf16": 1e-2}
+
+
+def rel_l2(x, ref):
+    x = np.asarray(x)
+    ref = np.asarray(ref)
+    return float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
+
+
+def band_limited_alm(lmax, mmax, nfield, seed):
+    """a_lm with C_l = (l+1)^-2; a_l0 real; zero for m > l (FP64)."""
+    rng = np.random.default_rng(seed)
+    l = np.arange(lmax)[:, None]
+    cl = (l + 1.0) ** -2
+    a = (rng.standard_normal((nfield, lmax, mmax)) + 1j * rng.standard_normal((nfield, lmax, mmax)))
+    a *= np.sqrt(cl / 2)
+    a[:, :, 0] = rng.standard_normal((nfield, lmax)) * np.sqrt(cl[:, 0])
+    m = np.arange(mmax)[None, :]
+    a[:, l[:, 0][:, None] < m] = 0
+    return a
+
+
+_oracles = {}
+
+
+def oracle(nside, lmax, mmax):
+    key = (nside, lmax, mmax)
+    if key not in _oracles:
+        _oracles.clear()
+        _oracles[key] = S.SHTOracle.healpix(nside, lmax, mmax)
+    return _oracles[key]
+
+
+@pytest.mark.parametrize("nside,lmax,mmax", [(1, 3, 3), (2, 5, 5), (4, 9, 9), (4, 15, 9), (8, 17, 17),
+                                             (8, 31, 17), (16, 20, 33), (32, 65, 65)])
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_forward_inverse_match_oracle(nside, lmax, mmax, precision):
+    o = oracle(nside, lmax, mmax)
+    rng = np.random.default_rng(11)
+    f = rng.standard_normal((3, o.grid.npix)).astype(np.float32)
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=lmax, mmax=mmax, precision=precision)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=lmax, mmax=mmax, precision=precision)
+    a = sht(torch.from_numpy(f).cuda()).cpu().numpy()
+    a_ref = o.forward(f.astype(np.float64))
+    assert rel_l2(a, a_ref) < TOL[precision], rel_l2(a, a_ref)
+    alm = band_limited_alm(lmax, mmax, 3, 5)
+    back = isht(torch.from_numpy(alm.astype(np.complex64)).cuda()).cpu().numpy()
+    f_ref = o.inverse(alm.astype(np.complex64).astype(np.complex128))
+    assert rel_l2(back, f_ref) < TOL[precision], rel_l2(back, f_ref)
+
+
+def test_fft_stage_folded_spectra():
+    """The workspace after the analysis stage holds, per m and ring pair k,
+    S = w (F_k + F_k') and D = w (F_k - F_k') (equator: S = D = w F)."""
+    nside, lmax, mmax = 8, 17, 17
+    o = oracle(nside, lmax, mmax)
+    plan = ShtPlan.healpix(nside, lmax, mmax)
+    n = 3
+    f = np.random.default_rng(3).standard_normal((n, o.grid.npix)).astype(np.float32)
+    ws, nbytes = plan.workspace(n)
+    out = torch.empty((n, lmax, mmax), dtype=torch.complex64, device="cuda")
+    from cuhpx_amd import _lib
+    rc = _lib.lib().hpx_sht_forward(plan.handle, 0, _lib.ptr(torch.from_numpy(f).cuda()), _lib.ptr(out), n,
+                                    _lib.ptr(ws), nbytes, _lib.stream_ptr())
+    _lib.check(rc, "forward")
+    torch.cuda.synchronize()
+    nring = o.grid.nring
+    npair = (nring + 1) // 2
+    kpad = (npair + 31) // 32 * 32
+    ncol = (2 * n + 31) // 32 * 32
+    X = ws[:mmax * 2 * kpad * ncol * 4].view(torch.float32).reshape(mmax, 2, kpad, ncol).cpu().numpy()
+    F = S.ring_rfft(f.astype(np.float64), o.grid, mmax) * o.grid.weight[:, None]
+    for k in range(npair):
+        ks = nring - 1 - k
+        s_ref = F[:, k] + (F[:, ks] if ks != k else 0)
+        d_ref = F[:, k] - (F[:, ks] if ks != k else 0)
+        if ks == k:
+            d_ref = F[:, k]
+        s = X[:, 0, k, 0:2 * n:2] + 1j * X[:, 0, k, 1:2 * n:2]
+        d = X[:, 1, k, 0:2 * n:2] + 1j * X[:, 1, k, 1:2 * n:2]
+        np.testing.assert_allclose(s.T, s_ref, atol=2e-6 * np.abs(F).max(), err_msg=f"S pair {k}")
+        np.testing.assert_allclose(d.T, d_ref, atol=2e-6 * np.abs(F).max(), err_msg=f"D pair {k}")
+
+
+def test_fp32_matches_reference_cpu_torch_path():
+    """The FP32 GPU path agrees with the reference's CPU torch path (restated,
+    float32, exact per-ring DFT) within 1e-5 relative L2 (config nside=32,
+    lmax=mmax=65, one map)."""
+    nside, L = 32, 65
+    g = torch.Generator().manual_seed(0)
+    f = torch.randn(12 * nside * nside, generator=g, dtype=torch.float64).to(torch.float32)
+    cpu = CPUTorchSHT(nside, L, L)
+    a_cpu = cpu.forward(f.clone()).numpy()
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=L)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=L)
+    a_gpu = sht(f.cuda())
+    assert a_gpu.shape == (L, L) and a_gpu.dtype == torch.complex64
+    assert rel_l2(a_gpu.cpu().numpy(), a_cpu) < 1e-5
+    back_cpu = cpu.inverse(torch.from_numpy(a_cpu).clone()).numpy()
+    back_gpu = isht(torch.from_numpy(a_cpu).cuda()).cpu().numpy()
+    assert rel_l2(back_gpu, back_cpu) < 1e-5
+
+
+def test_batch_equals_single_maps():
+    nside, L = 8, 17
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=L)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=L)
+    x = torch.randn(2, 3, 12 * nside * nside, device="cuda")
+    a = sht(x)
+    assert a.shape == (2, 3, L, L)
+    for i in range(2):
+        for j in range(3):
+            torch.testing.assert_close(a[i, j], sht(x[i, j]), rtol=0, atol=1e-6)
+    f = isht(a)
+    assert f.shape == x.shape
+    torch.testing.assert_close(f[1, 2], isht(a[1, 2]), rtol=0, atol=1e-6)
+
+
+def test_upper_triangle_is_zero_and_input_untouched():
+    nside = 4
+    sht = cuhpx_amd.SHTCUDA(nside)  # default lmax = 15, mmax = 9
+    isht = cuhpx_amd.iSHTCUDA(nside)
+    a = sht(torch.randn(12 * nside * nside, device="cuda"))
+    assert a.shape == (15, 9)
+    l = torch.arange(15)[:, None]
+    m = torch.arange(9)[None, :]
+    assert torch.all(a.cpu()[l < m] == 0)
+    a_copy = a.clone()
+    isht(a)
+    assert torch.equal(a, a_copy)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "bf16"])
+def test_nside256_band_limited_accuracy(precision):
+    """Config: nside 256, lmax = mmax = 513, band-limited fields synthesised in
+    FP64 (4 of the 64 fields), each direction against the FP64 oracle."""
+    nside, L = 256, 513
+    o = oracle(nside, L, L)
+    alm = band_limited_alm(L, L, 4, 1)
+    f64 = o.inverse(alm)
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=L, precision=precision)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=L, precision=precision)
+    f32 = f64.astype(np.float32)
+    a = sht(torch.from_numpy(f32).cuda()).cpu().numpy()
+    err_f = rel_l2(a, o.forward(f32.astype(np.float64)))
+    back = isht(torch.from_numpy(alm.astype(np.complex64)).cuda()).cpu().numpy()
+    err_i = rel_l2(back, o.inverse(alm.astype(np.complex64).astype(np.complex128)))
+    print(f"[nside256 {precision}] forward rel-L2 {err_f:.3e}  inverse rel-L2 {err_i:.3e}")
+    assert err_f < TOL[precision] and err_i < TOL[precision]
+
+
+@pytest.mark.parametrize("mode", ["exact", "compat"])
+def test_adjoints_match_oracle(mode):
+    nside, lmax, mmax = 8, 17, 17
+    o = oracle(nside, lmax, mmax)
+    rng = np.random.default_rng(9)
+    ga = (rng.standard_normal((2, lmax, mmax)) + 1j * rng.standard_normal((2, lmax, mmax)))
+    gf = rng.standard_normal((2, o.grid.npix))
+    plan = ShtPlan.healpix(nside, lmax, mmax)
+    got_f = plan.forward_adjoint(torch.from_numpy(ga.astype(np.complex64)).cuda(), "fp32", mode).cpu().numpy()
+    got_a = plan.inverse_adjoint(torch.from_numpy(gf.astype(np.float32)).cuda(), "fp32", mode).cpu().numpy()
+    ga32 = ga.astype(np.complex64).astype(np.complex128)
+    gf32 = gf.astype(np.float32).astype(np.float64)
+    if mode == "exact":
+        ref_f, ref_a = o.forward_adjoint(ga32), o.inverse_adjoint(gf32)
+    else:
+        ref_f, ref_a = o.forward_adjoint_compat(ga32), o.inverse_adjoint_compat(gf32)
+    assert rel_l2(got_f, ref_f) < 1e-5
+    assert rel_l2(got_a, ref_a) < 1e-5
+
+
+def test_autograd_through_modules():
+    nside, L = 8, 17
+    o = oracle(nside, L, L)
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=L)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=L)
+    rng = np.random.default_rng(4)
+    f = torch.from_numpy(rng.standard_normal(o.grid.npix).astype(np.float32)).cuda().requires_grad_(True)
+    w = torch.from_numpy(rng.standard_normal((L, L)).astype(np.float32)).cuda()
+    a = sht(f)
+    loss = (a.real * w).sum() + (a.imag * w).sum()
+    loss.backward()
+    ga = (w + 1j * w).cpu().numpy().astype(np.complex128)
+    assert rel_l2(f.grad.cpu().numpy(), o.forward_adjoint(ga)) < 1e-5
+    a2 = torch.from_numpy(band_limited_alm(L, L, 1, 2)[0].astype(np.complex64)).cuda().requires_grad_(True)
+    g = torch.from_numpy(rng.standard_normal(o.grid.npix).astype(np.float32)).cuda()
+    (isht(a2) * g).sum().backward()
+    ref = o.inverse_adjoint(g.cpu().numpy().astype(np.float64))
+    assert rel_l2(a2.grad.cpu().numpy(), ref) < 1e-5
+
+
+def test_errors():
+    sht = cuhpx_amd.SHTCUDA(4)
+    with pytest.raises(ValueError):
+        sht(torch.zeros(192, dtype=torch.complex64, device="cuda"))
+    with pytest.raises(TypeError):
+        sht(torch.zeros(192, dtype=torch.float64, device="cuda"))
+    with pytest.raises(RuntimeError):
+        sht(torch.zeros(191, device="cuda"))
