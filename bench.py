@@ -1,0 +1,269 @@
+#!/usr/bin/env python
+"""Forward + inverse SHT throughput on one MI355X.
+
+Workload: HEALPix nside = 256 (786432 pixels, RING order), lmax = mmax = 513,
+a batch of 64 band-limited fields (complex a_lm seeded Gaussian with
+C_l = (l+1)^-2, a_l0 real).  One step = forward SHT of the batch followed by
+the inverse SHT of the coefficients, with split-BF16 MFMA (3 terms, FP32
+accumulation) in the Legendre step and FP32 ring FFTs.  The plain FP32 path
+is timed beside it.  Inputs are resident in HBM before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--no-cpu-baseline]
+
+Prints one JSON line (rank 0).  Only one GPU is measured; --gpus N > 1 exits.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+NSIDE, LMAX, MMAX, BATCH = 256, 513, 513, 64
+PEAK = {"hbm_GBs": 8000.0, "bf16_TFLOPs": 2500.0, "fp32_TFLOPs": 157.3}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "bf16", "fp32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    return ap.parse_args()
+
+
+def band_limited_alm(lmax, mmax, nfield, seed):
+    g = torch.Generator().manual_seed(seed)
+    l = torch.arange(lmax, dtype=torch.float64)[:, None]
+    cl = (l + 1.0) ** -2
+    re = torch.randn(nfield, lmax, mmax, generator=g, dtype=torch.float64)
+    im = torch.randn(nfield, lmax, mmax, generator=g, dtype=torch.float64)
+    a = torch.complex(re, im) * torch.sqrt(cl / 2)
+    a[:, :, 0] = torch.randn(nfield, lmax, generator=g, dtype=torch.float64) * torch.sqrt(cl[:, 0])
+    mask = torch.arange(lmax)[:, None] < torch.arange(mmax)[None, :]
+    a[:, mask] = 0
+    return a
+
+
+def algorithmic_figures():
+    """Per-map algorithmic work, forward + inverse (one direction = half)."""
+    nring = 4 * NSIDE - 1
+    npix = 12 * NSIDE * NSIDE
+    n_half = 2 * NSIDE
+    n_lm = sum(LMAX - m for m in range(MMAX))
+    return {
+        "legendre_flop_per_map_per_dir": 4.0 * n_lm * n_half,
+        "fft_bytes_per_map_per_dir": 4.0 * npix + 8.0 * nring * MMAX,
+        "n_lm": n_lm,
+    }
+
+
+class Runner:
+    """Direct C-ABI loop over preallocated buffers (no allocation inside steps)."""
+
+    def __init__(self, plan, f, precision):
+        from cuhpx_amd import _lib
+        self.lib = _lib.lib()
+        self._lib = _lib
+        self.plan = plan
+        self.prec = _lib.PRECISIONS[precision]
+        self.f = f
+        self.n = f.shape[0]
+        self.alm = torch.empty((self.n, LMAX, MMAX), dtype=torch.complex64, device=f.device)
+        self.out = torch.empty_like(f)
+        self.ws, self.ws_bytes = plan.workspace(self.n)
+        self.args = dict(
+            h=plan.handle, f=_lib.ptr(self.f), alm=_lib.ptr(self.alm), out=_lib.ptr(self.out),
+            ws=_lib.ptr(self.ws))
+
+    def stage(self, i, stream):
+        a, L, p = self.args, self.lib, self.prec
+        if i == 0:
+            rc = L.hpx_sht_stage_ring_analysis(a["h"], a["f"], self.n, a["ws"], self.ws_bytes, stream)
+        elif i == 1:
+            rc = L.hpx_sht_stage_legendre_analysis(a["h"], p, a["alm"], self.n, a["ws"], self.ws_bytes, stream)
+        elif i == 2:
+            rc = L.hpx_sht_stage_legendre_synthesis(a["h"], p, a["alm"], self.n, a["ws"], self.ws_bytes, stream)
+        else:
+            rc = L.hpx_sht_stage_ring_synthesis(a["h"], a["out"], self.n, a["ws"], self.ws_bytes, stream)
+        if rc != 0:
+            self._lib.check(rc, f"stage {i}")
+
+    def time(self, steps, warmup):
+        stream = self._lib.stream_ptr(self.f.device)
+        for _ in range(warmup):
+            for i in range(4):
+                self.stage(i, stream)
+        torch.cuda.synchronize()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(steps)]
+        t0 = time.perf_counter()
+        for s in range(steps):
+            ev[s][0].record()
+            for i in range(4):
+                self.stage(i, stream)
+                ev[s][i + 1].record()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        stage_ms = [float(np.mean([ev[s][i].elapsed_time(ev[s][i + 1]) for s in range(steps)]))
+                    for i in range(4)]
+        return (t1 - t0) * 1e3 / steps, stage_ms
+
+
+def cpu_baseline(nmaps=3):
+    """The reference's CPU torch SHT path (restated in oracle/cpu_path.py) on
+    nmaps maps of the same workload, after one warm-up map."""
+    from oracle.cpu_path import CPUTorchSHT
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    t0 = time.perf_counter()
+    cpu = CPUTorchSHT(NSIDE, LMAX, MMAX)
+    build_s = time.perf_counter() - t0
+    alm = band_limited_alm(LMAX, MMAX, nmaps + 1, 1).to(torch.complex64)
+    times = []
+    for i in range(nmaps + 1):
+        t0 = time.perf_counter()
+        f = cpu.inverse(alm[i].clone())
+        cpu.forward(f)
+        if i > 0:
+            times.append(time.perf_counter() - t0)
+    per_map = float(np.median(times))
+    cpu_model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {
+        "value": 1.0 / per_map,
+        "unit": "maps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"{nmaps} maps (after 1 warm-up), forward+inverse each, median; reference CPU torch "
+                   f"path (per-ring torch.fft + einsum, float32) at nside={NSIDE}, lmax=mmax={LMAX}; "
+                   f"P-table build {build_s:.1f} s not timed; host '{cpu_model}'"),
+    }, cpu
+
+
+def accuracy(cpu, f32_in, alm_in, outputs):
+    """rel-L2 of GPU outputs vs the FP64 oracle on the first fields."""
+    from oracle.sht import SHTOracle
+    o = SHTOracle(cpu.grid, LMAX, MMAX, pct=cpu.pct.numpy())
+    f = f32_in.cpu().numpy().astype(np.float64)
+    a_ref = o.forward(f)
+    a = alm_in.cpu().numpy().astype(np.complex128)
+    f_ref = o.inverse(a)
+    res = {}
+    for name, (alm_out, f_out) in outputs.items():
+        ea = np.linalg.norm(alm_out.cpu().numpy() - a_ref) / np.linalg.norm(a_ref)
+        ef = np.linalg.norm(f_out.cpu().numpy() - f_ref) / np.linalg.norm(f_ref)
+        res[name] = {"forward_rel_l2": float(ea), "inverse_rel_l2": float(ef)}
+    return res
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one MI355X only (the SHT batch shards over GPUs with no "
+              "communication); run it with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(2)
+
+    from cuhpx_amd.hpx_sht import ShtPlan
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    t0 = time.perf_counter()
+    plan = ShtPlan.healpix(NSIDE, LMAX, MMAX)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+
+    alm0 = band_limited_alm(LMAX, MMAX, BATCH, 1).to(torch.complex64).to(dev)
+    f = plan.inverse(alm0, "fp32")  # band-limited synthetic fields, resident in HBM
+    torch.cuda.synchronize()
+
+    runs = {}
+    for prec in [args.precision, "fp32"] if args.precision != "fp32" else ["fp32"]:
+        r = Runner(plan, f, prec)
+        ms, stage_ms = r.time(args.steps, args.warmup)
+        runs[prec] = (ms, stage_ms, r)
+
+    ms, stage_ms, main_run = runs[args.precision]
+    value = BATCH / (ms / 1e3)
+    fig = algorithmic_figures()
+    names = ["ring_analysis", "legendre_analysis", "legendre_synthesis", "ring_synthesis"]
+    dom = int(np.argmax(stage_ms))
+    if dom in (1, 2):
+        flop = fig["legendre_flop_per_map_per_dir"] * BATCH
+        peak = PEAK["fp32_TFLOPs"] if args.precision == "fp32" else PEAK["bf16_TFLOPs"]
+        achieved = flop / (stage_ms[dom] * 1e-3) / 1e12
+        roof = {"kernel": names[dom], "bound": "mfma", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
+    else:
+        nbytes = fig["fft_bytes_per_map_per_dir"] * BATCH
+        achieved = nbytes / (stage_ms[dom] * 1e-3) / 1e9
+        roof = {"kernel": names[dom], "bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_GBs"],
+                "unit": "GB/s", "frac": achieved / PEAK["hbm_GBs"], "traffic": None}
+
+    result = {
+        "metric": "fwd+inv SHT maps/s, nside=256 batch 64: split-BF16 MFMA vs FP32 @ rel-L2<=2e-5",
+        "value": value,
+        "unit": "maps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if args.precision != "fp32" else "fp32",
+        "data": "synthetic (band-limited fields, C_l=(l+1)^-2, seeded)",
+        "config": {
+            "workload": "fwd+inv SHT, HEALPix nside=256 RING, lmax=mmax=513, batch 64",
+            "nside": NSIDE, "lmax": LMAX, "mmax": MMAX, "batch": BATCH,
+            "precision": args.precision, "parallelism": "single GPU",
+        },
+        "stages_ms": dict(zip(names, stage_ms)),
+        "roofline": roof,
+        "plan_build_s": plan_s,
+    }
+    if "fp32" in runs and args.precision != "fp32":
+        ms32 = runs["fp32"][0]
+        result["fp32_maps_per_s"] = BATCH / (ms32 / 1e3)
+        result["fp32_stages_ms"] = dict(zip(names, runs["fp32"][1]))
+        result["speedup_vs_fp32"] = ms32 / ms
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"], cpu = cpu_baseline()
+    if not args.no_accuracy:
+        if cpu is None:
+            from oracle.cpu_path import CPUTorchSHT
+            cpu = CPUTorchSHT(NSIDE, LMAX, MMAX)
+        nchk = 2
+        outs = {}
+        for prec, (_, _, r) in runs.items():
+            # the last timed step left alm = forward(f) and out = inverse(alm)
+            inv_of_ref = plan.inverse(alm0[:nchk].contiguous(), prec)
+            outs[prec] = (r.alm[:nchk], inv_of_ref)
+        result["accuracy_vs_fp64"] = accuracy(cpu, f[:nchk], alm0[:nchk], outs)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

@@ -7,11 +7,17 @@ hand-written HIP for gfx950 in libhpxmi.so, reached through a C ABI
 (include/hpxmi.h).
 """
 
-from . import sht_tools
+from . import cuhpx_fft, sht_tools
+from .hpx_regrid import Grid, InverseLatLonSHT, LatLonSHT, Regridding
 from .hpx_remap import flat2flat, flat2nest, flat2ring, nest2flat, nest2ring, ring2flat, ring2nest
 from .hpx_sht import SHTCUDA, ShtPlan, iSHTCUDA
 
 __all__ = [
+    "Grid",
+    "Regridding",
+    "LatLonSHT",
+    "InverseLatLonSHT",
+    "cuhpx_fft",
     "SHTCUDA",
     "iSHTCUDA",
     "ShtPlan",

@@ -60,6 +60,12 @@ SIGNATURES = {
     "hpx_sht_inverse": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_forward_adjoint": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_inverse_adjoint": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_ring_rfft": (_c_int, [_vp, _vp, _vp, _c_i64, _vp]),
+    "hpx_ring_irfft": (_c_int, [_vp, _vp, _vp, _c_i64, _vp]),
+    "hpx_sht_stage_ring_analysis": (_c_int, [_vp, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_sht_stage_legendre_analysis": (_c_int, [_vp, _c_int, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_sht_stage_legendre_synthesis": (_c_int, [_vp, _c_int, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_sht_stage_ring_synthesis": (_c_int, [_vp, _vp, _c_i64, _vp, _c_i64, _vp]),
 }
 
 

@@ -182,7 +182,7 @@ __global__ __launch_bounds__(kThreads) void ring_analysis_kernel(
     const float2* __restrict__ phase, const float2* __restrict__ chirp_all,
     const float2* __restrict__ bhat_all, const float2* __restrict__ tw, int log2_tab,
     int64_t npix, int64_t nbatch, int mmax, int kpad, int64_t ncol, int use_weight,
-    float scale, int cmode) {
+    float scale, int cmode, int layout, int nring) {
   __shared__ float2 buf[kFftLds];
   const int k = blockIdx.x;
   const RingPair pr = pairs[k];
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void ring_analysis_kernel(
   const bool blue = pr.M != N;
   const float2* chirp = chirp_all + pr.chirp_off;
   const float2* bhat = bhat_all + pr.bhat_off;
-  int log2G = kFftLds == 8192 ? 13 - pr.log2M : 0;
+  int log2G = 13 - pr.log2M;
   if (log2G > 3) log2G = 3;
   const int G = 1 << log2G;
   const int64_t b_begin = (int64_t)blockIdx.y * kFieldsPerFftBlock;
@@ -220,10 +220,13 @@ __global__ __launch_bounds__(kThreads) void ring_analysis_kernel(
     ring_dft(buf, pr, total, bhat, tw, log2_tab);
 
     for (int e = threadIdx.x; e < (mmax << log2G); e += kThreads) {
-      const int m = e >> log2G;
-      const int g = e & (G - 1);
+      // folded layout: fields fastest (consecutive fields are adjacent
+      // columns); ring-major layout: m fastest (contiguous along m)
+      const int m = layout == 0 ? e >> log2G : e % mmax;
+      const int g = layout == 0 ? e & (G - 1) : e / mmax;
       if (g >= gcount) continue;
       float2 S = make_float2(0.f, 0.f), D = make_float2(0.f, 0.f);
+      float2 Fn = S, Fs = S;
       if (m < K) {
         const float2 zm = dft_value(buf, pr, g, m, chirp);
         const float2 zc = dft_value(buf, pr, g, m == 0 ? 0 : N - m, chirp);
@@ -236,17 +239,25 @@ __global__ __launch_bounds__(kThreads) void ring_analysis_kernel(
         xs = cmul(xs, ph);
         float c = wscale;
         if (cmode == 1 && m != 0 && 2 * m != N) c *= 2.0f;
+        Fn = make_float2(c * xn.x, c * xn.y);
+        Fs = make_float2(c * xs.x, c * xs.y);
         if (pr.single) {
-          S = make_float2(c * xn.x, c * xn.y);
+          S = Fn;
           D = S;
         } else {
-          S = make_float2(c * (xn.x + xs.x), c * (xn.y + xs.y));
-          D = make_float2(c * (xn.x - xs.x), c * (xn.y - xs.y));
+          S = make_float2(Fn.x + Fs.x, Fn.y + Fs.y);
+          D = make_float2(Fn.x - Fs.x, Fn.y - Fs.y);
         }
       }
-      float* base = X + (int64_t)m * mstride + (int64_t)k * ncol + 2 * (b0 + g);
-      *reinterpret_cast<float2*>(base) = S;
-      *reinterpret_cast<float2*>(base + pstride) = D;
+      if (layout == 0) {
+        float* base = X + (int64_t)m * mstride + (int64_t)k * ncol + 2 * (b0 + g);
+        *reinterpret_cast<float2*>(base) = S;
+        *reinterpret_cast<float2*>(base + pstride) = D;
+      } else {
+        float2* ftm = reinterpret_cast<float2*>(X) + (b0 + g) * nring * (int64_t)mmax;
+        ftm[(int64_t)k * mmax + m] = Fn;
+        if (!pr.single) ftm[(int64_t)(nring - 1 - k) * mmax + m] = Fs;
+      }
     }
     __syncthreads();
   }
@@ -257,7 +268,7 @@ __global__ __launch_bounds__(kThreads) void ring_synthesis_kernel(
     const float2* __restrict__ phase, const float2* __restrict__ chirp_all,
     const float2* __restrict__ bhat_all, const float2* __restrict__ tw, int log2_tab,
     int64_t npix, int64_t nbatch, int mmax, int kpad, int64_t ncol, int use_weight,
-    float scale, int cmode) {
+    float scale, int cmode, int layout, int nring) {
   __shared__ float2 buf[kFftLds];
   const int k = blockIdx.x;
   const RingPair pr = pairs[k];
@@ -285,16 +296,24 @@ __global__ __launch_bounds__(kThreads) void ring_synthesis_kernel(
       const int m = e >> log2G;
       const int g = e & (G - 1);
       if (g >= gcount) continue;
-      const float* base = X + (int64_t)m * mstride + (int64_t)k * ncol + 2 * (b0 + g);
-      const float2 S = *reinterpret_cast<const float2*>(base);
-      const float2 D = *reinterpret_cast<const float2*>(base + pstride);
+      float2 Gn, Gs;
+      if (layout == 0) {
+        const float* base = X + (int64_t)m * mstride + (int64_t)k * ncol + 2 * (b0 + g);
+        const float2 S = *reinterpret_cast<const float2*>(base);
+        const float2 D = *reinterpret_cast<const float2*>(base + pstride);
+        Gn = make_float2(S.x + D.x, S.y + D.y);
+        Gs = make_float2(S.x - D.x, S.y - D.y);
+      } else {
+        const float2* ftm = reinterpret_cast<const float2*>(X) + (b0 + g) * nring * (int64_t)mmax;
+        Gn = ftm[(int64_t)k * mmax + m];
+        Gs = pr.single ? Gn : ftm[(int64_t)(nring - 1 - k) * mmax + m];
+      }
       const float2 ph = cconj(phase[(int64_t)k * mmax + m]);
       const bool edge = (m == 0) || (2 * m == N);
       float c = wscale;
       if (cmode == 1 && !edge) c *= 0.5f;
-      float2 xn = cmul(make_float2(c * (S.x + D.x), c * (S.y + D.y)), ph);
-      float2 xs = pr.single ? make_float2(0.f, 0.f)
-                            : cmul(make_float2(c * (S.x - D.x), c * (S.y - D.y)), ph);
+      float2 xn = cmul(make_float2(c * Gn.x, c * Gn.y), ph);
+      float2 xs = pr.single ? make_float2(0.f, 0.f) : cmul(make_float2(c * Gs.x, c * Gs.y), ph);
       if (edge) {
         xn.y = 0.f;
         xs.y = 0.f;
@@ -329,27 +348,29 @@ __global__ __launch_bounds__(kThreads) void ring_synthesis_kernel(
 }  // namespace
 
 int launch_analysis(const ShtPlan& p, const float* f, float* X, int64_t nbatch, int use_weight,
-                    float scale, int cmode, hipStream_t s) {
+                    float scale, int cmode, hipStream_t s, int layout) {
   if (nbatch <= 0) return HPX_OK;
   const int64_t ygroups = (nbatch + kFieldsPerFftBlock - 1) / kFieldsPerFftBlock;
   if (ygroups > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large (%lld)", (long long)nbatch);
   dim3 grid(p.npair, (unsigned)ygroups);
   ring_analysis_kernel<<<grid, kThreads, 0, s>>>(f, X, p.d_pairs, p.d_phase, p.d_chirp, p.d_bhat,
                                                  p.d_twiddle, p.log2_max_M, p.npix, nbatch, p.mmax,
-                                                 p.kpad, col_pad(nbatch), use_weight, scale, cmode);
+                                                 p.kpad, col_pad(nbatch), use_weight, scale, cmode,
+                                                 layout, p.nring);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
 
 int launch_synthesis(const ShtPlan& p, const float* X, float* f, int64_t nbatch, int use_weight,
-                     float scale, int cmode, hipStream_t s) {
+                     float scale, int cmode, hipStream_t s, int layout) {
   if (nbatch <= 0) return HPX_OK;
   const int64_t ygroups = (nbatch + kFieldsPerFftBlock - 1) / kFieldsPerFftBlock;
   if (ygroups > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large (%lld)", (long long)nbatch);
   dim3 grid(p.npair, (unsigned)ygroups);
   ring_synthesis_kernel<<<grid, kThreads, 0, s>>>(X, f, p.d_pairs, p.d_phase, p.d_chirp, p.d_bhat,
                                                   p.d_twiddle, p.log2_max_M, p.npix, nbatch, p.mmax,
-                                                  p.kpad, col_pad(nbatch), use_weight, scale, cmode);
+                                                  p.kpad, col_pad(nbatch), use_weight, scale, cmode,
+                                                 layout, p.nring);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }

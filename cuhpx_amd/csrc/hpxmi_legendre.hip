@@ -528,7 +528,7 @@ int launch_legendre_fwd(const ShtPlan& p, int precision, const float* X, float* 
     }
     HPX_LAUNCH_CHECK();
   }
-  if (p.mmax > 1) {
+  if (p.mmax > 1 && p.lmax > 0) {
     zero_upper_kernel<<<(unsigned)(nbatch * p.lmax), kThreads, 0, s>>>(alm, p.lmax, p.mmax, nbatch);
     HPX_LAUNCH_CHECK();
   }

@@ -100,7 +100,7 @@ void free_plan(ShtPlan* p) {
 int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const double* phi0,
                const double* weights, int lmax, int mmax, hipStream_t stream, ShtPlan** out) {
   if (nring < 1) HPX_FAIL(HPX_ERR_ARG, "nring must be >= 1");
-  if (lmax < 1 || mmax < 1) HPX_FAIL(HPX_ERR_ARG, "lmax and mmax must be >= 1 (got %d, %d)", lmax, mmax);
+  if (lmax < 0 || mmax < 1) HPX_FAIL(HPX_ERR_ARG, "lmax must be >= 0 and mmax >= 1 (got %d, %d)", lmax, mmax);
   if (mmax > 65535 || lmax > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "lmax/mmax too large");
   for (int t = 0; t < nring; ++t) {
     const int u = nring - 1 - t;
@@ -261,6 +261,7 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
 
 int check_call(const ShtPlan* p, int precision, int64_t nbatch, int64_t ws_bytes) {
   if (p == nullptr) HPX_FAIL(HPX_ERR_ARG, "null plan");
+  if (p->lmax < 1) HPX_FAIL(HPX_ERR_ARG, "plan was built with lmax = 0 (ring FFTs only)");
   if (precision < 0 || precision > 2) HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
   if (nbatch < 0) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 0");
   const int64_t need = workspace_floats(*p, nbatch) * (int64_t)sizeof(float);
@@ -397,4 +398,58 @@ extern "C" int hpx_sht_inverse_adjoint(const hpx_sht_plan* plan, int precision, 
   else rc = launch_analysis(*pl, gf, X, nbatch, 0, 2.0f, 0, stream);
   if (rc != HPX_OK) return rc;
   return launch_legendre_fwd(*pl, precision, X, galm, nbatch, stream);
+}
+
+extern "C" int hpx_sht_stage_ring_analysis(const hpx_sht_plan* plan, const float* f, int64_t nbatch,
+                                           void* workspace, int64_t workspace_bytes,
+                                           hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  int rc = check_call(pl, 0, nbatch, workspace_bytes);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  return launch_analysis(*pl, f, static_cast<float*>(workspace), nbatch, 1, 1.0f, 0, stream);
+}
+
+extern "C" int hpx_sht_stage_legendre_analysis(const hpx_sht_plan* plan, int precision, float* alm,
+                                               int64_t nbatch, void* workspace,
+                                               int64_t workspace_bytes, hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  int rc = check_call(pl, precision, nbatch, workspace_bytes);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  return launch_legendre_fwd(*pl, precision, static_cast<const float*>(workspace), alm, nbatch, stream);
+}
+
+extern "C" int hpx_sht_stage_legendre_synthesis(const hpx_sht_plan* plan, int precision,
+                                                const float* alm, int64_t nbatch, void* workspace,
+                                                int64_t workspace_bytes, hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  int rc = check_call(pl, precision, nbatch, workspace_bytes);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  return launch_legendre_inv(*pl, precision, alm, static_cast<float*>(workspace), nbatch, stream);
+}
+
+extern "C" int hpx_sht_stage_ring_synthesis(const hpx_sht_plan* plan, float* f, int64_t nbatch,
+                                            void* workspace, int64_t workspace_bytes,
+                                            hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  int rc = check_call(pl, 0, nbatch, workspace_bytes);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  return launch_synthesis(*pl, static_cast<const float*>(workspace), f, nbatch, 0, 1.0f, 0, stream);
+}
+
+extern "C" int hpx_ring_rfft(const hpx_sht_plan* plan, const float* f, float* ftm, int64_t nbatch,
+                             hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  if (pl == nullptr) HPX_FAIL(HPX_ERR_ARG, "null plan");
+  if (nbatch < 0) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 0");
+  if (nbatch == 0) return HPX_OK;
+  return launch_analysis(*pl, f, ftm, nbatch, 0, 1.0f, 0, stream, 1);
+}
+
+extern "C" int hpx_ring_irfft(const hpx_sht_plan* plan, const float* ftm, float* f, int64_t nbatch,
+                              hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  if (pl == nullptr) HPX_FAIL(HPX_ERR_ARG, "null plan");
+  if (nbatch < 0) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 0");
+  if (nbatch == 0) return HPX_OK;
+  return launch_synthesis(*pl, ftm, f, nbatch, 0, 1.0f, 0, stream, 1);
 }

@@ -70,10 +70,12 @@ inline int64_t workspace_floats(const ShtPlan& p, int64_t nbatch) {
 }
 
 // Launchers (return HPX_OK or an error code after setting the error text).
+// layout 0: folded spectra workspace X[m][parity][kpad][ncol];
+// layout 1: ring-major complex ftm[nbatch][nring][mmax] (no folding).
 int launch_analysis(const ShtPlan& p, const float* f, float* X, int64_t nbatch,
-                    int use_weight, float scale, int cmode, hipStream_t s);
+                    int use_weight, float scale, int cmode, hipStream_t s, int layout = 0);
 int launch_synthesis(const ShtPlan& p, const float* X, float* f, int64_t nbatch,
-                     int use_weight, float scale, int cmode, hipStream_t s);
+                     int use_weight, float scale, int cmode, hipStream_t s, int layout = 0);
 int launch_legendre_fwd(const ShtPlan& p, int precision, const float* X, float* alm,
                         int64_t nbatch, hipStream_t s);
 int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float* X,

@@ -163,6 +163,43 @@ int hpx_sht_inverse_adjoint(const hpx_sht_plan* plan, int precision, int mode,
                             void* workspace, int64_t workspace_bytes,
                             hipStream_t stream);
 
+/* Per-ring real FFTs alone, in the reference's ring-major layout
+ * ftm[nbatch, nring, mmax] (complex64), where mmax is the plan's mmax
+ * (the reference's L).  A plan built with lmax = 0 has no Legendre table and
+ * serves only these two calls.
+ * hpx_ring_rfft replaces cuhpx_fft.healpix_rfft_batch / healpix_rfft_class
+ * (src/harmonic_transform/hpx_fft.cpp:21-84, 152-193):
+ *   ftm[b, t, m] = e^{-i m phi0_t} sum_j f[b, off_t + j] e^{-2 pi i m j / N_t}
+ *   for m < min(mmax, N_t/2 + 1), 0 above.
+ * hpx_ring_irfft replaces cuhpx_fft.healpix_irfft_batch / healpix_irfft_class
+ * (hpx_fft.cpp:86-149, 195-238): per ring, irfft(ftm[b, t, :] e^{i m phi0_t},
+ * n = N_t, norm = "forward"); unlike the reference it leaves ftm unchanged. */
+int hpx_ring_rfft(const hpx_sht_plan* plan, const float* f, float* ftm, int64_t nbatch,
+                  hipStream_t stream);
+int hpx_ring_irfft(const hpx_sht_plan* plan, const float* ftm, float* f, int64_t nbatch,
+                   hipStream_t stream);
+
+/* Split-phase entry points: the two stages of each direction, with the
+ * ring-pair spectra left in the workspace between them (layout
+ * [mmax][2][kpad][ncol] float32, kpad = ring pairs rounded up to 32,
+ * ncol = 2*nbatch rounded up to 32; parity 0 holds S = w (F_t + F_t'),
+ * parity 1 holds D = w (F_t - F_t')).  hpx_sht_forward is
+ * hpx_sht_stage_ring_analysis followed by hpx_sht_stage_legendre_analysis;
+ * hpx_sht_inverse is hpx_sht_stage_legendre_synthesis followed by
+ * hpx_sht_stage_ring_synthesis.  No counterpart in the reference (its stages
+ * are the cuhpx_fft calls and the einsums inside one autograd Function);
+ * they let callers time, overlap or pipeline the stages. */
+int hpx_sht_stage_ring_analysis(const hpx_sht_plan* plan, const float* f, int64_t nbatch,
+                                void* workspace, int64_t workspace_bytes, hipStream_t stream);
+int hpx_sht_stage_legendre_analysis(const hpx_sht_plan* plan, int precision, float* alm,
+                                    int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                                    hipStream_t stream);
+int hpx_sht_stage_legendre_synthesis(const hpx_sht_plan* plan, int precision, const float* alm,
+                                     int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                                     hipStream_t stream);
+int hpx_sht_stage_ring_synthesis(const hpx_sht_plan* plan, float* f, int64_t nbatch,
+                                 void* workspace, int64_t workspace_bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

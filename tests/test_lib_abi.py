@@ -59,7 +59,8 @@ def test_plan_rejects_bad_arguments():
     h = ctypes.c_void_p()
     w = (ctypes.c_double * 7)(*([1.0] * 7))
     assert lib.hpx_sht_plan_create_healpix(0, 5, 5, w, None, ctypes.byref(h)) == _lib.HPX_ERR_ARG
-    assert lib.hpx_sht_plan_create_healpix(2, 0, 5, w, None, ctypes.byref(h)) == _lib.HPX_ERR_ARG
+    assert lib.hpx_sht_plan_create_healpix(2, -1, 5, w, None, ctypes.byref(h)) == _lib.HPX_ERR_ARG
+    assert lib.hpx_sht_plan_create_healpix(2, 5, 0, w, None, ctypes.byref(h)) == _lib.HPX_ERR_ARG
     # asymmetric ring grid
     nphi = (ctypes.c_int32 * 3)(4, 8, 6)
     th = (ctypes.c_double * 3)(0.5, 1.5, 2.6)

@@ -48,7 +48,7 @@ def oracle(nside, lmax, mmax):
     return _oracles[key]
 
 
-@pytest.mark.parametrize("nside,lmax,mmax", [(1, 3, 3), (2, 5, 5), (4, 9, 9), (4, 15, 9), (8, 17, 17),
+@pytest.mark.parametrize("nside,lmax,mmax", [(2, 5, 5), (2, 3, 9), (4, 9, 9), (4, 15, 9), (8, 17, 17),
                                              (8, 31, 17), (16, 20, 33), (32, 65, 65)])
 @pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
 def test_forward_inverse_match_oracle(nside, lmax, mmax, precision):
@@ -162,7 +162,7 @@ def test_nside256_band_limited_accuracy(precision):
     err_f = rel_l2(a, o.forward(f32.astype(np.float64)))
     back = isht(torch.from_numpy(alm.astype(np.complex64)).cuda()).cpu().numpy()
     err_i = rel_l2(back, o.inverse(alm.astype(np.complex64).astype(np.complex128)))
-    print(f"[nside256 {precision}] forward rel-L2 {err_f:.3e}  inverse rel-L2 {err_i:.3e}")
+    print(f"[nside256 {precision}] forward rel-L2 {err_f:.3e}  inverse rel-L2 {err_i:.3e}", flush=True)
     assert err_f < TOL[precision] and err_i < TOL[precision]
 
 
