@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "bf16", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--remaps", action="store_true",
+                    help="measure the pixel remaps at nside=1024, [64, npix] float32 instead")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="measure the plain copy kernel (HBM yardstick) instead")
     return ap.parse_args()
 
 
@@ -174,6 +178,57 @@ def accuracy(cpu, f32_in, alm_in, outputs):
     return res
 
 
+def time_events(fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    ev[0].record()
+    for i in range(steps):
+        fn()
+        ev[i + 1].record()
+    torch.cuda.synchronize()
+    return float(np.mean([ev[i].elapsed_time(ev[i + 1]) for i in range(steps)]))
+
+
+def bench_calibrate(args):
+    """Plain 16-byte-vector copy of 4 GiB: the HBM yardstick."""
+    from cuhpx_amd import _lib
+    n = 4 << 30
+    src = torch.empty(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    stream = _lib.stream_ptr()
+    fn = lambda: _lib.check(_lib.lib().hpx_copy(_lib.ptr(src), _lib.ptr(dst), n, stream), "copy")
+    ms = time_events(fn, args.steps, args.warmup)
+    print(json.dumps({"metric": "plain copy HBM bandwidth", "value": 2 * n / (ms * 1e-3) / 1e9,
+                      "unit": "GB/s", "bytes_per_launch": 2 * n, "ms_per_launch": ms,
+                      "peak_GBs": PEAK["hbm_GBs"]}))
+
+
+def bench_remaps(args):
+    """nest2ring / ring2nest / nest2flat / flat2flat on [64, 12*1024^2] float32."""
+    import cuhpx_amd
+    nside, rows = 1024, 64
+    npix = 12 * nside * nside
+    g = torch.Generator(device="cuda").manual_seed(2)
+    x = torch.randn(rows, npix, device="cuda", generator=g)
+    out = {}
+    for name, fn in [("nest2ring", lambda: cuhpx_amd.nest2ring(x, nside)),
+                     ("ring2nest", lambda: cuhpx_amd.ring2nest(x, nside)),
+                     ("nest2flat", lambda: cuhpx_amd.nest2flat(x, "N", True, nside)),
+                     ("flat2flat", lambda: cuhpx_amd.flat2flat(x, "S", False, "W", True, nside))]:
+        ms = time_events(fn, args.steps, args.warmup)
+        gbs = 8.0 * rows * npix / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "GB_s": gbs, "frac_of_peak": gbs / PEAK["hbm_GBs"]}
+    worst = min(v["GB_s"] for v in out.values())
+    print(json.dumps({"metric": "remap HBM GB/s, nside=1024, [64, npix] float32 (min over kinds)",
+                      "value": worst, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+                      "warmup": args.warmup, "higher_is_better": True, "dtype": "f32",
+                      "data": "synthetic N(0,1)", "kinds": out,
+                      "roofline": {"bound": "hbm", "achieved": worst, "peak": PEAK["hbm_GBs"],
+                                   "unit": "GB/s", "frac": worst / PEAK["hbm_GBs"], "traffic": None}}))
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -188,6 +243,10 @@ def main():
     from cuhpx_amd.hpx_sht import ShtPlan
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calibrate:
+        return bench_calibrate(args)
+    if args.remaps:
+        return bench_remaps(args)
     t0 = time.perf_counter()
     plan = ShtPlan.healpix(NSIDE, LMAX, MMAX)
     torch.cuda.synchronize()

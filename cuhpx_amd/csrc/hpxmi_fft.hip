@@ -9,18 +9,23 @@
 //   f[off_t + j] = sum_{m < K_t} c_m Re(G_t[m] e^{i m (phi0_t + 2 pi j / N_t)})
 //   with irfft(norm="forward") semantics (hpx_fft_cuda.cu:1213-1332).
 //
-// MI355X design.  One work-group owns one mirror pair of rings (north t and
-// south nring-1-t have the same length, phase and weight) and 8 fields.  The
-// two real rings of a pair are packed into one complex sequence z = f_t + i
-// f_t', so one complex DFT serves both.  The DFT runs entirely in LDS: a
-// radix-8/4/2 Stockham FFT (each pass: registers <- LDS, barrier, twiddle +
-// butterfly, LDS <- registers, barrier).  Rings whose length is not a power
-// of two use Bluestein's chirp-z algorithm with a padded power-of-two length
-// M >= 2N-1 and FP64-built chirp / kernel-spectrum tables.  Chirp multiply,
-// phase shift, quadrature weight, the split into the two rings and the
-// equatorial folding S = F_t + F_t', D = F_t - F_t' (the Legendre step's
-// operands for l+m even / odd) are fused into the load and store loops, so
-// the only HBM traffic is the map and the folded spectra.
+// MI355X design.  The two real rings of a mirror pair (north t and south
+// nring-1-t share length, phase and weight) are packed into one complex
+// sequence z = f_t + i f_t', so one complex DFT serves both.  Each DFT is
+// owned by one wavefront and runs in that wave's slice of LDS: a radix-8/4
+// Stockham FFT whose passes need no work-group barrier (LDS accesses of one
+// wavefront complete in program order), so the waves of a work-group progress
+// independently.  Rings whose length is not a power of two use Bluestein's
+// chirp-z algorithm with a padded power-of-two length M >= 2N-1 and FP64-built
+// chirp / kernel-spectrum tables.  A work-group holds one ring pair and one
+// wave per field; pairs are launched in classes of equal maximum FFT length so
+// that the LDS allocation fits the class.  Chirp multiply, phase shift,
+// quadrature weight, the split into the two rings and the equatorial folding
+// S = F_t + F_t', D = F_t - F_t' (the Legendre step's operands for l+m even /
+// odd) are fused into the load and store loops; the folded spectra are
+// written through LDS so that consecutive lanes write consecutive fields.
+// Rings that need an 8192-point FFT (nside > 512) use a work-group-wide
+// variant of the same Stockham passes.
 
 #include "hpxmi_sht_internal.h"
 #include "../../include/hpxmi.h"
@@ -28,14 +33,13 @@
 namespace hpxmi {
 namespace {
 
-constexpr int kThreads = 512;
-
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(s * a.x, s * a.y); }
 // multiply by -i
 __device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
@@ -79,47 +83,442 @@ __device__ __forceinline__ void dft<8>(float2* v) {
   }
 }
 
-// One radix-R Stockham pass over `total` points = G sequences of length M
-// (M = 2^log2M) stored back to back in buf.  Ns is the length of the
-// sub-transforms already combined.  Twiddles come from the table
-// tw[x] = exp(-2 pi i x / 2^log2_tab).
 template <int R>
-__device__ __forceinline__ void stockham_pass(float2* buf, int log2M, int total, int Ns,
-                                              int log2Ns, const float2* __restrict__ tw,
-                                              int log2_tab) {
-  constexpr int kMaxB = kFftLds / R / kThreads;
-  constexpr int kLogR = R == 8 ? 3 : (R == 4 ? 2 : 1);
-  const int nb = total / R;
+constexpr int log2_of() {
+  return R == 8 ? 3 : (R == 4 ? 2 : 1);
+}
+
+// v[r] *= w^r for r = 1..R-1 (powers of the table entry by multiplication).
+template <int R>
+__device__ __forceinline__ void twiddle(float2* v, float2 w1) {
+  float2 w = w1;
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    v[r] = cmul(v[r], w);
+    if (r + 1 < R) w = cmul(w, w1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level FFT: one wavefront, one sequence of M = 2^log2M points in LDS,
+// natural order in and out.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void wave_sync() {
+  // The LDS accesses of one wavefront are performed in program order; this
+  // only keeps the compiler from moving them across a pass boundary.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One radix-R Stockham pass over M = 2^LOG2M points by one wavefront, after
+// 2^LOG2NS points have been combined.  All strides are compile-time constants,
+// so the LDS accesses use immediate offsets.
+template <int R, int LOG2M, int LOG2NS>
+__device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict__ tw, int log2_tab,
+                                          int lane) {
+  constexpr int kLogR = log2_of<R>();
+  constexpr int kNB = 1 << (LOG2M - kLogR);  // butterflies; also the input stride
+  constexpr int kNS = 1 << LOG2NS;
+  constexpr int kB = (kNB + 63) / 64;
+  const int tw_shift = log2_tab - LOG2NS - kLogR;
+  float2 v[kB][R];
+  float2 w[kB];
+#pragma unroll
+  for (int i = 0; i < kB; ++i) {
+    const int b = lane + 64 * i;
+    if (kNS > 1 && (kNB >= 64 || b < kNB)) w[i] = tw[(b & (kNS - 1)) << tw_shift];
+  }
+#pragma unroll
+  for (int i = 0; i < kB; ++i) {
+    const int b = lane + 64 * i;
+    if (kNB >= 64 || b < kNB) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[i][r] = buf[b + r * kNB];
+    }
+  }
+  wave_sync();
+#pragma unroll
+  for (int i = 0; i < kB; ++i) {
+    const int b = lane + 64 * i;
+    if (kNB >= 64 || b < kNB) {
+      const int jn = b & (kNS - 1);
+      if (kNS > 1) twiddle<R>(v[i], w[i]);
+      dft<R>(v[i]);
+      float2* dst = buf + (b - jn) * R + jn;
+#pragma unroll
+      for (int r = 0; r < R; ++r) dst[r * kNS] = v[i][r];
+    }
+  }
+  wave_sync();
+}
+
+// Radix schedule: radix 4 for 2 or 4 remaining bits, radix 2 only for M = 2,
+// radix 8 otherwise.
+template <int LOG2M, int LOG2NS = 0>
+__device__ __forceinline__ void wave_fft_fixed(float2* buf, const float2* __restrict__ tw,
+                                               int log2_tab, int lane) {
+  if constexpr (LOG2NS < LOG2M) {
+    constexpr int rem = LOG2M - LOG2NS;
+    constexpr int R = rem == 1 ? 2 : ((rem == 2 || rem == 4) ? 4 : 8);
+    wave_pass<R, LOG2M, LOG2NS>(buf, tw, log2_tab, lane);
+    wave_fft_fixed<LOG2M, LOG2NS + log2_of<R>()>(buf, tw, log2_tab, lane);
+  }
+}
+
+// Forward DFT of 2^log2M points (natural order in and out) by one wavefront,
+// for the sizes of one size class (log2M in [LO, HI]).
+template <int LO, int HI>
+__device__ __forceinline__ void wave_fft(float2* buf, int log2M, const float2* __restrict__ tw,
+                                         int log2_tab, int lane) {
+  if constexpr (LO <= HI) {
+    if (log2M == LO) {
+      wave_fft_fixed<LO>(buf, tw, log2_tab, lane);
+      return;
+    }
+    wave_fft<LO + 1, HI>(buf, log2M, tw, log2_tab, lane);
+  }
+}
+
+// DFT of length N of the wave's sequence (entered as z_j, or z_j * chirp_j
+// zero-padded to M for Bluestein).  Afterwards the DFT value at x < N is
+// dft_value(buf, ...).  Bluestein: X[m] = chirp_m conj(c_m),
+// c = FFT(conj(FFT(a) * bhat)) with bhat = FFT(conj chirp) / M.
+template <int LO, int HI>
+__device__ __forceinline__ void wave_ring_dft(float2* buf, const RingPair& pr,
+                                              const float2* __restrict__ bhat,
+                                              const float2* __restrict__ tw, int log2_tab, int lane) {
+  wave_fft<LO, HI>(buf, pr.log2M, tw, log2_tab, lane);
+  if (pr.M != pr.nphi) {
+    // all loads of the kernel spectrum in flight before the LDS updates
+    constexpr int kIt = ((1 << HI) + 63) / 64;
+    float2 bh[kIt];
+#pragma unroll
+    for (int i = 0; i < kIt; ++i) {
+      const int x = lane + 64 * i;
+      if (x < pr.M) bh[i] = bhat[x];
+    }
+#pragma unroll
+    for (int i = 0; i < kIt; ++i) {
+      const int x = lane + 64 * i;
+      if (x < pr.M) buf[x] = cconj(cmul(buf[x], bh[i]));
+    }
+    wave_sync();
+    wave_fft<LO, HI>(buf, pr.log2M, tw, log2_tab, lane);
+  }
+}
+
+__device__ __forceinline__ float2 dft_value(const float2* seq, const RingPair& pr, int x,
+                                            const float2* __restrict__ chirp) {
+  const float2 v = seq[x];
+  if (pr.M == pr.nphi) return v;
+  return cmul(chirp[x], cconj(v));
+}
+
+// The two rings' spectra at m from the packed DFT Z (zm = Z[m], zc = Z[N-m]):
+// Fn = (Z[m] + conj Z[N-m]) / 2, Fs = (Z[m] - conj Z[N-m]) / (2i), each times
+// the phase factor and the scale c.
+__device__ __forceinline__ void unpack_pair(float2 zm, float2 zc, float2 ph, float c, float2& Fn,
+                                            float2& Fs) {
+  const float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y - zc.y));
+  const float2 xs = make_float2(0.5f * (zm.y + zc.y), 0.5f * (zc.x - zm.x));
+  Fn = cscale(cmul(xn, ph), c);
+  Fs = cscale(cmul(xs, ph), c);
+}
+
+// Synthesis input at m for the two rings' spectra Gn, Gs: the DFT below is
+// applied to conj(Z) with Z = Xn + i Xs Hermitian-extended; writes positions
+// m and (unless m is DC or Nyquist) N - m of the sequence (chirp-multiplied
+// for Bluestein).
+__device__ __forceinline__ void pack_pair(float2* seq, const RingPair& pr, int m, float2 Gn,
+                                          float2 Gs, float2 ph_conj, float c, int cmode,
+                                          const float2* __restrict__ chirp) {
+  const int N = pr.nphi;
+  const bool edge = (m == 0) || (2 * m == N);
+  if (cmode == 1 && !edge) c *= 0.5f;
+  float2 xn = cmul(cscale(Gn, c), ph_conj);
+  float2 xs = pr.single ? make_float2(0.f, 0.f) : cmul(cscale(Gs, c), ph_conj);
+  if (edge) {
+    xn.y = 0.f;
+    xs.y = 0.f;
+  }
+  const bool blue = pr.M != N;
+  float2 zp = cconj(make_float2(xn.x - xs.y, xn.y + xs.x));
+  if (blue) zp = cmul(zp, chirp[m]);
+  seq[m] = zp;
+  if (!edge) {
+    float2 zn = cconj(make_float2(xn.x + xs.y, xs.x - xn.y));
+    if (blue) zn = cmul(zn, chirp[N - m]);
+    seq[N - m] = zn;
+  }
+}
+
+struct StageArgs {
+  const RingPair* pairs;
+  const int* plist;       // pair indices of this class
+  const float2* phase;    // [npair][mmax]
+  const float2* chirp;
+  const float2* bhat;
+  const float2* tw;
+  int log2_tab;
+  int64_t npix, nbatch;
+  int mmax, kpad, nring;
+  int64_t ncol;
+  int use_weight, cmode, layout;
+  float scale;
+};
+
+// ---------------------------------------------------------------------------
+// Wave-level kernels: work-group = one ring pair x WAVES fields
+// ---------------------------------------------------------------------------
+
+template <int WAVES, int LO, int HI>
+__global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const float* __restrict__ f,
+                                                                        float* __restrict__ X,
+                                                                        StageArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int kThreadsW = WAVES * 64;
+  const int64_t nfg = (a.nbatch + WAVES - 1) / WAVES;
+  const int64_t item = xcd_local_index(blockIdx.x, gridDim.x);
+  const int k = a.plist[item / nfg];
+  const int64_t b0 = (item % nfg) * WAVES;
+  const RingPair pr = a.pairs[k];
+  const int N = pr.nphi, M = pr.M;
+  const bool blue = M != N;
+  const float2* chirp = a.chirp + pr.chirp_off;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t b = b0 + wave;
+  float2* buf = lds + wave * M;
+  if (b < a.nbatch) {
+    const float* fn = f + b * a.npix + pr.off_n;
+    const float* fs = f + b * a.npix + (pr.single ? pr.off_n : pr.off_s);
+    constexpr int kIt = ((1 << HI) + 63) / 64;
+    constexpr int kBatch = kIt < 8 ? kIt : 8;
+#pragma unroll
+    for (int i0 = 0; i0 < kIt; i0 += kBatch) {
+      float2 z[kBatch], c[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int j = lane + 64 * (i0 + u);
+        z[u] = make_float2(0.f, 0.f);
+        if (j < N) {
+          z[u].x = fn[j];
+          z[u].y = fs[j];
+          if (blue) c[u] = chirp[j];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int j = lane + 64 * (i0 + u);
+        if (j < M) {
+          float2 v = z[u];
+          if (pr.single) v.y = 0.f;
+          if (blue && j < N) v = cmul(v, c[u]);
+          buf[j] = v;
+        }
+      }
+    }
+    wave_sync();
+    wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab, lane);
+  }
+  const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
+  const int K = min(a.mmax, N / 2 + 1);
+  const float2* ph = a.phase + (int64_t)k * a.mmax;
+  if (a.layout == 0) {
+    // folded spectra X[m][parity][k][2b + re/im]: fields fastest so that
+    // consecutive lanes write consecutive columns
+    __syncthreads();
+    const int64_t pstride = (int64_t)a.kpad * a.ncol;
+    const int total = a.mmax * WAVES;
+    constexpr int kBatch = 4;
+    for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * kThreadsW) {
+      // loads of this thread's next kBatch elements first (phase and chirp
+      // tables), then the arithmetic and stores
+      float2 phv[kBatch], cm[kBatch], cc[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kThreadsW;
+        const int m = e / WAVES;
+        if (e < total && m < K) {
+          phv[u] = ph[m];
+          if (blue) {
+            cm[u] = chirp[m];
+            cc[u] = chirp[m == 0 ? 0 : N - m];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kThreadsW;
+        if (e >= total) continue;
+        const int m = e / WAVES;
+        const int g = e - m * WAVES;
+        if (b0 + g >= a.nbatch) continue;
+        float2 S = make_float2(0.f, 0.f), D = S;
+        if (m < K) {
+          const float2* seq = lds + g * M;
+          float2 zm = seq[m], zc = seq[m == 0 ? 0 : N - m];
+          if (blue) {
+            zm = cmul(cm[u], cconj(zm));
+            zc = cmul(cc[u], cconj(zc));
+          }
+          const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
+          float2 Fn, Fs;
+          unpack_pair(zm, zc, phv[u], c, Fn, Fs);
+          if (pr.single) {
+            S = Fn;
+            D = Fn;
+          } else {
+            S = cadd(Fn, Fs);
+            D = csub(Fn, Fs);
+          }
+        }
+        float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * (b0 + g);
+        *reinterpret_cast<float2*>(base) = S;
+        *reinterpret_cast<float2*>(base + pstride) = D;
+      }
+    }
+  } else if (b < a.nbatch) {
+    // ring-major ftm[b][t][m]: lanes along m
+    float2* out = reinterpret_cast<float2*>(X) + b * a.nring * (int64_t)a.mmax;
+    for (int m = lane; m < a.mmax; m += 64) {
+      float2 Fn = make_float2(0.f, 0.f), Fs = Fn;
+      if (m < K) {
+        const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
+        unpack_pair(dft_value(buf, pr, m, chirp), dft_value(buf, pr, m == 0 ? 0 : N - m, chirp),
+                    ph[m], c, Fn, Fs);
+      }
+      out[(int64_t)k * a.mmax + m] = Fn;
+      if (!pr.single) out[(int64_t)(a.nring - 1 - k) * a.mmax + m] = Fs;
+    }
+  }
+}
+
+template <int WAVES, int LO, int HI>
+__global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const float* __restrict__ X,
+                                                                         float* __restrict__ f,
+                                                                         StageArgs a) {
+  extern __shared__ float2 lds[];
+  constexpr int kThreadsW = WAVES * 64;
+  const int64_t nfg = (a.nbatch + WAVES - 1) / WAVES;
+  const int64_t item = xcd_local_index(blockIdx.x, gridDim.x);
+  const int k = a.plist[item / nfg];
+  const int64_t b0 = (item % nfg) * WAVES;
+  const RingPair pr = a.pairs[k];
+  const int N = pr.nphi, M = pr.M;
+  const float2* chirp = a.chirp + pr.chirp_off;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t b = b0 + wave;
+  const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
+  const int K = min(a.mmax, N / 2 + 1);
+  const float2* ph = a.phase + (int64_t)k * a.mmax;
+
+  for (int idx = threadIdx.x; idx < WAVES * M; idx += kThreadsW) lds[idx] = make_float2(0.f, 0.f);
+  __syncthreads();
+  if (a.layout == 0) {
+    const int64_t pstride = (int64_t)a.kpad * a.ncol;
+    const int total = K * WAVES;
+    constexpr int kBatch = 4;
+    for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * kThreadsW) {
+      float2 S[kBatch], D[kBatch], phv[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kThreadsW;
+        const int m = e / WAVES;
+        const int g = e - m * WAVES;
+        if (e < total && b0 + g < a.nbatch) {
+          const float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * (b0 + g);
+          S[u] = *reinterpret_cast<const float2*>(base);
+          D[u] = *reinterpret_cast<const float2*>(base + pstride);
+          phv[u] = ph[m];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int e = e0 + u * kThreadsW;
+        const int m = e / WAVES;
+        const int g = e - m * WAVES;
+        if (e < total && b0 + g < a.nbatch)
+          pack_pair(lds + g * M, pr, m, cadd(S[u], D[u]), csub(S[u], D[u]), cconj(phv[u]), wscale,
+                    a.cmode, chirp);
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < K * WAVES; e += kThreadsW) {
+      const int g = e / K;
+      const int m = e - g * K;
+      if (b0 + g >= a.nbatch) continue;
+      const float2* in = reinterpret_cast<const float2*>(X) + (b0 + g) * a.nring * (int64_t)a.mmax;
+      const float2 Gn = in[(int64_t)k * a.mmax + m];
+      const float2 Gs = pr.single ? Gn : in[(int64_t)(a.nring - 1 - k) * a.mmax + m];
+      pack_pair(lds + g * M, pr, m, Gn, Gs, cconj(ph[m]), wscale, a.cmode, chirp);
+    }
+  }
+  __syncthreads();
+  if (b >= a.nbatch) return;
+  float2* buf = lds + wave * M;
+  wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab, lane);
+  float* fr = f + b * a.npix;
+  constexpr int kIt = ((1 << HI) + 63) / 64;
+  constexpr int kBatch = kIt < 8 ? kIt : 8;
+  const bool blue = M != N;
+#pragma unroll
+  for (int i0 = 0; i0 < kIt; i0 += kBatch) {
+    float2 c[kBatch];
+    if (blue) {
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int j = lane + 64 * (i0 + u);
+        if (j < N) c[u] = chirp[j];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int j = lane + 64 * (i0 + u);
+      if (j < N) {
+        float2 r = buf[j];
+        if (blue) r = cmul(c[u], cconj(r));
+        fr[pr.off_n + j] = r.x;
+        if (!pr.single) fr[pr.off_s + j] = -r.y;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Work-group-wide FFT for the 8192-point class: one ring pair x one field per
+// work-group, Stockham passes across 512 threads with barriers.
+// ---------------------------------------------------------------------------
+
+constexpr int kBlockThreads = 512;
+
+template <int R>
+__device__ __forceinline__ void block_pass(float2* buf, int log2M, int Ns, int log2Ns,
+                                           const float2* __restrict__ tw, int log2_tab) {
+  constexpr int kLogR = log2_of<R>();
+  constexpr int kMaxB = (kFftLds / R + kBlockThreads - 1) / kBlockThreads;
   const int log2mr = log2M - kLogR;
-  const int mr = 1 << log2mr;
-  const int M = 1 << log2M;
+  const int nb = 1 << log2mr;
   const int tw_shift = log2_tab - log2Ns - kLogR;
   float2 v[kMaxB][R];
 #pragma unroll
   for (int i = 0; i < kMaxB; ++i) {
-    const int b = threadIdx.x + i * kThreads;
+    const int b = threadIdx.x + i * kBlockThreads;
     if (b < nb) {
-      const int g = b >> log2mr;
-      const int j = b & (mr - 1);
-      const float2* src = buf + g * M + j;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = src[r * mr];
+      for (int r = 0; r < R; ++r) v[i][r] = buf[b + (r << log2mr)];
     }
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kMaxB; ++i) {
-    const int b = threadIdx.x + i * kThreads;
+    const int b = threadIdx.x + i * kBlockThreads;
     if (b < nb) {
-      const int g = b >> log2mr;
-      const int j = b & (mr - 1);
-      const int jn = j & (Ns - 1);
-      if (Ns > 1) {
-#pragma unroll
-        for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[(jn * r) << tw_shift]);
-      }
+      const int jn = b & (Ns - 1);
+      if (Ns > 1) twiddle<R>(v[i], tw[jn << tw_shift]);
       dft<R>(v[i]);
-      float2* dst = buf + g * M + (j - jn) * R + jn;
+      float2* dst = buf + (b - jn) * R + jn;
 #pragma unroll
       for (int r = 0; r < R; ++r) dst[r * Ns] = v[i][r];
     }
@@ -127,252 +526,206 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int log2M, int total,
   __syncthreads();
 }
 
-// Forward DFT (exp(-2 pi i ...)) of G sequences of length 2^log2M in LDS,
-// natural order in and out.  Ends with a barrier.
-__device__ void lds_fft(float2* buf, int log2M, int total, const float2* __restrict__ tw,
-                        int log2_tab) {
+__device__ void block_fft(float2* buf, int log2M, const float2* __restrict__ tw, int log2_tab) {
   int Ns = 1, log2Ns = 0, rem = log2M;
   while (rem > 0) {
-    if (rem >= 3) {
-      stockham_pass<8>(buf, log2M, total, Ns, log2Ns, tw, log2_tab);
-      Ns <<= 3;
-      log2Ns += 3;
-      rem -= 3;
-    } else if (rem == 2) {
-      stockham_pass<4>(buf, log2M, total, Ns, log2Ns, tw, log2_tab);
-      Ns <<= 2;
-      log2Ns += 2;
-      rem -= 2;
-    } else {
-      stockham_pass<2>(buf, log2M, total, Ns, log2Ns, tw, log2_tab);
+    if (rem == 1) {
+      block_pass<2>(buf, log2M, Ns, log2Ns, tw, log2_tab);
+      rem -= 1;
       Ns <<= 1;
       log2Ns += 1;
-      rem -= 1;
+    } else if (rem == 2 || rem == 4) {
+      block_pass<4>(buf, log2M, Ns, log2Ns, tw, log2_tab);
+      rem -= 2;
+      Ns <<= 2;
+      log2Ns += 2;
+    } else {
+      block_pass<8>(buf, log2M, Ns, log2Ns, tw, log2_tab);
+      rem -= 3;
+      Ns <<= 3;
+      log2Ns += 3;
     }
   }
 }
 
-// The DFT of length N of each sequence in buf, entered as z_j (pow2, M == N)
-// or z_j * chirp_j zero-padded to M (Bluestein).  On return, for j < N the
-// DFT value is dft_value(buf, ...).  Bluestein: X[m] = chirp_m conj(c_m),
-// c = FFT(conj(FFT(a) * bhat)) with bhat = FFT(conj chirp)/M.
-__device__ void ring_dft(float2* buf, const RingPair& pr, int total, const float2* __restrict__ bhat,
-                         const float2* __restrict__ tw, int log2_tab) {
-  lds_fft(buf, pr.log2M, total, tw, log2_tab);
+__device__ void block_ring_dft(float2* buf, const RingPair& pr, const float2* __restrict__ bhat,
+                               const float2* __restrict__ tw, int log2_tab) {
+  block_fft(buf, pr.log2M, tw, log2_tab);
   if (pr.M != pr.nphi) {
-    const int M = pr.M;
-    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
-      const int x = idx & (M - 1);
-      buf[idx] = cconj(cmul(buf[idx], bhat[x]));
-    }
+    for (int x = threadIdx.x; x < pr.M; x += kBlockThreads) buf[x] = cconj(cmul(buf[x], bhat[x]));
     __syncthreads();
-    lds_fft(buf, pr.log2M, total, tw, log2_tab);
+    block_fft(buf, pr.log2M, tw, log2_tab);
   }
 }
 
-__device__ __forceinline__ float2 dft_value(const float2* buf, const RingPair& pr, int g, int x,
-                                            const float2* __restrict__ chirp) {
-  const float2 v = buf[g * pr.M + x];
-  if (pr.M == pr.nphi) return v;
-  return cmul(chirp[x], cconj(v));
-}
-
-__global__ __launch_bounds__(kThreads) void ring_analysis_kernel(
-    const float* __restrict__ f, float* __restrict__ X, const RingPair* __restrict__ pairs,
-    const float2* __restrict__ phase, const float2* __restrict__ chirp_all,
-    const float2* __restrict__ bhat_all, const float2* __restrict__ tw, int log2_tab,
-    int64_t npix, int64_t nbatch, int mmax, int kpad, int64_t ncol, int use_weight,
-    float scale, int cmode, int layout, int nring) {
+__global__ __launch_bounds__(kBlockThreads) void ring_analysis_block_kernel(
+    const float* __restrict__ f, float* __restrict__ X, StageArgs a) {
   __shared__ float2 buf[kFftLds];
-  const int k = blockIdx.x;
-  const RingPair pr = pairs[k];
-  const int N = pr.nphi;
-  const bool blue = pr.M != N;
-  const float2* chirp = chirp_all + pr.chirp_off;
-  const float2* bhat = bhat_all + pr.bhat_off;
-  int log2G = 13 - pr.log2M;
-  if (log2G > 3) log2G = 3;
-  const int G = 1 << log2G;
-  const int64_t b_begin = (int64_t)blockIdx.y * kFieldsPerFftBlock;
-  int64_t b_end = b_begin + kFieldsPerFftBlock;
-  if (b_end > nbatch) b_end = nbatch;
-  const float wscale = (use_weight ? pr.weight : 1.0f) * scale;
-  const int K = min(mmax, N / 2 + 1);
-  const int64_t pstride = (int64_t)kpad * ncol;
-  const int64_t mstride = 2 * pstride;
-  const int total = G << pr.log2M;
-
-  for (int64_t b0 = b_begin; b0 < b_end; b0 += G) {
-    const int gcount = (int)min((int64_t)G, b_end - b0);
-    for (int idx = threadIdx.x; idx < total; idx += kThreads) {
-      const int g = idx >> pr.log2M;
-      const int j = idx & (pr.M - 1);
-      float2 z = make_float2(0.f, 0.f);
-      if (g < gcount && j < N) {
-        const float* fr = f + (b0 + g) * npix;
-        z.x = fr[pr.off_n + j];
-        z.y = pr.single ? 0.f : fr[pr.off_s + j];
-        if (blue) z = cmul(z, chirp[j]);
-      }
-      buf[idx] = z;
+  const int64_t item = blockIdx.x;
+  const int k = a.plist[item / a.nbatch];
+  const int64_t b = item % a.nbatch;
+  const RingPair pr = a.pairs[k];
+  const int N = pr.nphi, M = pr.M;
+  const bool blue = M != N;
+  const float2* chirp = a.chirp + pr.chirp_off;
+  const float* fr = f + b * a.npix;
+  for (int j = threadIdx.x; j < M; j += kBlockThreads) {
+    float2 z = make_float2(0.f, 0.f);
+    if (j < N) {
+      z.x = fr[pr.off_n + j];
+      z.y = pr.single ? 0.f : fr[pr.off_s + j];
+      if (blue) z = cmul(z, chirp[j]);
     }
-    __syncthreads();
-    ring_dft(buf, pr, total, bhat, tw, log2_tab);
-
-    for (int e = threadIdx.x; e < (mmax << log2G); e += kThreads) {
-      // folded layout: fields fastest (consecutive fields are adjacent
-      // columns); ring-major layout: m fastest (contiguous along m)
-      const int m = layout == 0 ? e >> log2G : e % mmax;
-      const int g = layout == 0 ? e & (G - 1) : e / mmax;
-      if (g >= gcount) continue;
-      float2 S = make_float2(0.f, 0.f), D = make_float2(0.f, 0.f);
-      float2 Fn = S, Fs = S;
-      if (m < K) {
-        const float2 zm = dft_value(buf, pr, g, m, chirp);
-        const float2 zc = dft_value(buf, pr, g, m == 0 ? 0 : N - m, chirp);
-        // DFTs of the two real rings: Xn = (Z[m] + conj Z[N-m]) / 2,
-        // Xs = (Z[m] - conj Z[N-m]) / (2i).
-        float2 xn = make_float2(0.5f * (zm.x + zc.x), 0.5f * (zm.y - zc.y));
-        float2 xs = make_float2(0.5f * (zm.y + zc.y), 0.5f * (zc.x - zm.x));
-        const float2 ph = phase[(int64_t)k * mmax + m];
-        xn = cmul(xn, ph);
-        xs = cmul(xs, ph);
-        float c = wscale;
-        if (cmode == 1 && m != 0 && 2 * m != N) c *= 2.0f;
-        Fn = make_float2(c * xn.x, c * xn.y);
-        Fs = make_float2(c * xs.x, c * xs.y);
-        if (pr.single) {
-          S = Fn;
-          D = S;
-        } else {
-          S = make_float2(Fn.x + Fs.x, Fn.y + Fs.y);
-          D = make_float2(Fn.x - Fs.x, Fn.y - Fs.y);
-        }
-      }
-      if (layout == 0) {
-        float* base = X + (int64_t)m * mstride + (int64_t)k * ncol + 2 * (b0 + g);
-        *reinterpret_cast<float2*>(base) = S;
-        *reinterpret_cast<float2*>(base + pstride) = D;
-      } else {
-        float2* ftm = reinterpret_cast<float2*>(X) + (b0 + g) * nring * (int64_t)mmax;
-        ftm[(int64_t)k * mmax + m] = Fn;
-        if (!pr.single) ftm[(int64_t)(nring - 1 - k) * mmax + m] = Fs;
-      }
+    buf[j] = z;
+  }
+  __syncthreads();
+  block_ring_dft(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab);
+  const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
+  const int K = min(a.mmax, N / 2 + 1);
+  const float2* ph = a.phase + (int64_t)k * a.mmax;
+  const int64_t pstride = (int64_t)a.kpad * a.ncol;
+  for (int m = threadIdx.x; m < a.mmax; m += kBlockThreads) {
+    float2 Fn = make_float2(0.f, 0.f), Fs = Fn;
+    if (m < K) {
+      const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
+      unpack_pair(dft_value(buf, pr, m, chirp), dft_value(buf, pr, m == 0 ? 0 : N - m, chirp), ph[m],
+                  c, Fn, Fs);
     }
-    __syncthreads();
+    if (a.layout == 0) {
+      const float2 S = pr.single ? Fn : cadd(Fn, Fs);
+      const float2 D = pr.single ? Fn : csub(Fn, Fs);
+      float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * b;
+      *reinterpret_cast<float2*>(base) = S;
+      *reinterpret_cast<float2*>(base + pstride) = D;
+    } else {
+      float2* out = reinterpret_cast<float2*>(X) + b * a.nring * (int64_t)a.mmax;
+      out[(int64_t)k * a.mmax + m] = Fn;
+      if (!pr.single) out[(int64_t)(a.nring - 1 - k) * a.mmax + m] = Fs;
+    }
   }
 }
 
-__global__ __launch_bounds__(kThreads) void ring_synthesis_kernel(
-    const float* __restrict__ X, float* __restrict__ f, const RingPair* __restrict__ pairs,
-    const float2* __restrict__ phase, const float2* __restrict__ chirp_all,
-    const float2* __restrict__ bhat_all, const float2* __restrict__ tw, int log2_tab,
-    int64_t npix, int64_t nbatch, int mmax, int kpad, int64_t ncol, int use_weight,
-    float scale, int cmode, int layout, int nring) {
+__global__ __launch_bounds__(kBlockThreads) void ring_synthesis_block_kernel(
+    const float* __restrict__ X, float* __restrict__ f, StageArgs a) {
   __shared__ float2 buf[kFftLds];
-  const int k = blockIdx.x;
-  const RingPair pr = pairs[k];
-  const int N = pr.nphi;
-  const bool blue = pr.M != N;
-  const float2* chirp = chirp_all + pr.chirp_off;
-  const float2* bhat = bhat_all + pr.bhat_off;
-  int log2G = 13 - pr.log2M;
-  if (log2G > 3) log2G = 3;
-  const int G = 1 << log2G;
-  const int64_t b_begin = (int64_t)blockIdx.y * kFieldsPerFftBlock;
-  int64_t b_end = b_begin + kFieldsPerFftBlock;
-  if (b_end > nbatch) b_end = nbatch;
-  const float wscale = (use_weight ? pr.weight : 1.0f) * scale;
-  const int K = min(mmax, N / 2 + 1);
-  const int64_t pstride = (int64_t)kpad * ncol;
-  const int64_t mstride = 2 * pstride;
-  const int total = G << pr.log2M;
-
-  for (int64_t b0 = b_begin; b0 < b_end; b0 += G) {
-    const int gcount = (int)min((int64_t)G, b_end - b0);
-    for (int idx = threadIdx.x; idx < total; idx += kThreads) buf[idx] = make_float2(0.f, 0.f);
-    __syncthreads();
-    for (int e = threadIdx.x; e < (K << log2G); e += kThreads) {
-      const int m = e >> log2G;
-      const int g = e & (G - 1);
-      if (g >= gcount) continue;
-      float2 Gn, Gs;
-      if (layout == 0) {
-        const float* base = X + (int64_t)m * mstride + (int64_t)k * ncol + 2 * (b0 + g);
-        const float2 S = *reinterpret_cast<const float2*>(base);
-        const float2 D = *reinterpret_cast<const float2*>(base + pstride);
-        Gn = make_float2(S.x + D.x, S.y + D.y);
-        Gs = make_float2(S.x - D.x, S.y - D.y);
-      } else {
-        const float2* ftm = reinterpret_cast<const float2*>(X) + (b0 + g) * nring * (int64_t)mmax;
-        Gn = ftm[(int64_t)k * mmax + m];
-        Gs = pr.single ? Gn : ftm[(int64_t)(nring - 1 - k) * mmax + m];
-      }
-      const float2 ph = cconj(phase[(int64_t)k * mmax + m]);
-      const bool edge = (m == 0) || (2 * m == N);
-      float c = wscale;
-      if (cmode == 1 && !edge) c *= 0.5f;
-      float2 xn = cmul(make_float2(c * Gn.x, c * Gn.y), ph);
-      float2 xs = pr.single ? make_float2(0.f, 0.f) : cmul(make_float2(c * Gs.x, c * Gs.y), ph);
-      if (edge) {
-        xn.y = 0.f;
-        xs.y = 0.f;
-      }
-      // Z = Xn + i Xs at m and its Hermitian partner at N - m; the DFT below
-      // is applied to conj(Z) (an inverse DFT by conjugation).
-      float2 zp = make_float2(xn.x - xs.y, xn.y + xs.x);
-      zp = cconj(zp);
-      if (blue) zp = cmul(zp, chirp[m]);
-      buf[g * pr.M + m] = zp;
-      if (!edge) {
-        float2 zn = make_float2(xn.x + xs.y, xs.x - xn.y);
-        zn = cconj(zn);
-        if (blue) zn = cmul(zn, chirp[N - m]);
-        buf[g * pr.M + (N - m)] = zn;
-      }
+  const int64_t item = blockIdx.x;
+  const int k = a.plist[item / a.nbatch];
+  const int64_t b = item % a.nbatch;
+  const RingPair pr = a.pairs[k];
+  const int N = pr.nphi, M = pr.M;
+  const float2* chirp = a.chirp + pr.chirp_off;
+  const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
+  const int K = min(a.mmax, N / 2 + 1);
+  const float2* ph = a.phase + (int64_t)k * a.mmax;
+  for (int idx = threadIdx.x; idx < M; idx += kBlockThreads) buf[idx] = make_float2(0.f, 0.f);
+  __syncthreads();
+  const int64_t pstride = (int64_t)a.kpad * a.ncol;
+  for (int m = threadIdx.x; m < K; m += kBlockThreads) {
+    float2 Gn, Gs;
+    if (a.layout == 0) {
+      const float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * b;
+      const float2 S = *reinterpret_cast<const float2*>(base);
+      const float2 D = *reinterpret_cast<const float2*>(base + pstride);
+      Gn = cadd(S, D);
+      Gs = csub(S, D);
+    } else {
+      const float2* in = reinterpret_cast<const float2*>(X) + b * a.nring * (int64_t)a.mmax;
+      Gn = in[(int64_t)k * a.mmax + m];
+      Gs = pr.single ? Gn : in[(int64_t)(a.nring - 1 - k) * a.mmax + m];
     }
-    __syncthreads();
-    ring_dft(buf, pr, total, bhat, tw, log2_tab);
-    for (int g = 0; g < gcount; ++g) {
-      float* fr = f + (b0 + g) * npix;
-      for (int j = threadIdx.x; j < N; j += kThreads) {
-        const float2 r = dft_value(buf, pr, g, j, chirp);
-        fr[pr.off_n + j] = r.x;
-        if (!pr.single) fr[pr.off_s + j] = -r.y;
-      }
-    }
-    __syncthreads();
+    pack_pair(buf, pr, m, Gn, Gs, cconj(ph[m]), wscale, a.cmode, chirp);
   }
+  __syncthreads();
+  block_ring_dft(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab);
+  float* fr = f + b * a.npix;
+  for (int j = threadIdx.x; j < N; j += kBlockThreads) {
+    const float2 r = dft_value(buf, pr, j, chirp);
+    fr[pr.off_n + j] = r.x;
+    if (!pr.single) fr[pr.off_s + j] = -r.y;
+  }
+}
+
+StageArgs make_args(const ShtPlan& p, int64_t nbatch, int use_weight, float scale, int cmode,
+                    int layout) {
+  StageArgs a;
+  a.pairs = p.d_pairs;
+  a.plist = p.d_plist;
+  a.phase = p.d_phase;
+  a.chirp = p.d_chirp;
+  a.bhat = p.d_bhat;
+  a.tw = p.d_twiddle;
+  a.log2_tab = p.log2_max_M;
+  a.npix = p.npix;
+  a.nbatch = nbatch;
+  a.mmax = p.mmax;
+  a.kpad = p.kpad;
+  a.nring = p.nring;
+  a.ncol = col_pad(nbatch);
+  a.use_weight = use_weight;
+  a.cmode = cmode;
+  a.layout = layout;
+  a.scale = scale;
+  return a;
+}
+
+template <bool ANALYSIS>
+int launch_stage(const ShtPlan& p, const float* in, float* out, int64_t nbatch, int use_weight,
+                 float scale, int cmode, hipStream_t s, int layout) {
+  if (nbatch <= 0) return HPX_OK;
+  for (int c = 0; c < kFftClasses; ++c) {
+    const int n = p.class_n[c];
+    if (n == 0) continue;
+    StageArgs a = make_args(p, nbatch, use_weight, scale, cmode, layout);
+    a.plist = p.d_plist + p.class_off[c];
+    if (c == kFftClasses - 1) {
+      const int64_t items = (int64_t)n * nbatch;
+      if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+      if (ANALYSIS)
+        ring_analysis_block_kernel<<<(unsigned)items, kBlockThreads, 0, s>>>(in, out, a);
+      else
+        ring_synthesis_block_kernel<<<(unsigned)items, kBlockThreads, 0, s>>>(in, out, a);
+      HPX_LAUNCH_CHECK();
+      continue;
+    }
+    // class c < 3: FFT length <= 2^(10 + c); waves (= fields) per work-group
+    // chosen so that the work-group's LDS is 64 KB
+    const int waves = c == 0 ? 8 : (c == 1 ? 4 : 2);
+    const int64_t nfg = (nbatch + waves - 1) / waves;
+    const int64_t items = (int64_t)n * nfg;
+    if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+    const size_t lds_bytes = ((size_t)waves << (10 + c)) * sizeof(float2);
+    const unsigned grid = (unsigned)items;
+    if (ANALYSIS) {
+      if (c == 0) ring_analysis_wave_kernel<8, 1, 10><<<grid, 8 * 64, lds_bytes, s>>>(in, out, a);
+      else if (c == 1) ring_analysis_wave_kernel<4, 11, 11><<<grid, 4 * 64, lds_bytes, s>>>(in, out, a);
+      else ring_analysis_wave_kernel<2, 12, 12><<<grid, 2 * 64, lds_bytes, s>>>(in, out, a);
+    } else {
+      if (c == 0) ring_synthesis_wave_kernel<8, 1, 10><<<grid, 8 * 64, lds_bytes, s>>>(in, out, a);
+      else if (c == 1) ring_synthesis_wave_kernel<4, 11, 11><<<grid, 4 * 64, lds_bytes, s>>>(in, out, a);
+      else ring_synthesis_wave_kernel<2, 12, 12><<<grid, 2 * 64, lds_bytes, s>>>(in, out, a);
+    }
+    HPX_LAUNCH_CHECK();
+  }
+  return HPX_OK;
 }
 
 }  // namespace
 
+int fft_class_of(int log2M) {
+  if (log2M <= 10) return 0;
+  if (log2M == 11) return 1;
+  if (log2M == 12) return 2;
+  return 3;
+}
+
 int launch_analysis(const ShtPlan& p, const float* f, float* X, int64_t nbatch, int use_weight,
                     float scale, int cmode, hipStream_t s, int layout) {
-  if (nbatch <= 0) return HPX_OK;
-  const int64_t ygroups = (nbatch + kFieldsPerFftBlock - 1) / kFieldsPerFftBlock;
-  if (ygroups > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large (%lld)", (long long)nbatch);
-  dim3 grid(p.npair, (unsigned)ygroups);
-  ring_analysis_kernel<<<grid, kThreads, 0, s>>>(f, X, p.d_pairs, p.d_phase, p.d_chirp, p.d_bhat,
-                                                 p.d_twiddle, p.log2_max_M, p.npix, nbatch, p.mmax,
-                                                 p.kpad, col_pad(nbatch), use_weight, scale, cmode,
-                                                 layout, p.nring);
-  HPX_LAUNCH_CHECK();
-  return HPX_OK;
+  return launch_stage<true>(p, f, X, nbatch, use_weight, scale, cmode, s, layout);
 }
 
 int launch_synthesis(const ShtPlan& p, const float* X, float* f, int64_t nbatch, int use_weight,
                      float scale, int cmode, hipStream_t s, int layout) {
-  if (nbatch <= 0) return HPX_OK;
-  const int64_t ygroups = (nbatch + kFieldsPerFftBlock - 1) / kFieldsPerFftBlock;
-  if (ygroups > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large (%lld)", (long long)nbatch);
-  dim3 grid(p.npair, (unsigned)ygroups);
-  ring_synthesis_kernel<<<grid, kThreads, 0, s>>>(X, f, p.d_pairs, p.d_phase, p.d_chirp, p.d_bhat,
-                                                  p.d_twiddle, p.log2_max_M, p.npix, nbatch, p.mmax,
-                                                  p.kpad, col_pad(nbatch), use_weight, scale, cmode,
-                                                 layout, p.nring);
-  HPX_LAUNCH_CHECK();
-  return HPX_OK;
+  return launch_stage<false>(p, X, f, nbatch, use_weight, scale, cmode, s, layout);
 }
 
 }  // namespace hpxmi

@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
     const int64_t* __restrict__ p_off, const int* __restrict__ nl,
     const int2* __restrict__ items, int nitems, int npair, int kpad, int lmax, int mmax,
     int64_t ncol, int64_t nbatch) {
-  __shared__ FwdLds<PREC> lds;
+  __shared__ FwdLds<PREC> lds[2];
   const int item = xcd_local_index(blockIdx.x, nitems);
   const int m = items[item].x;
   const int jb = items[item].y;
@@ -180,94 +180,128 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
   const bool srow_ok = srow < (sp ? padp1 : padp0);
   const float* Psrc = Pm + (sp ? (int64_t)padp0 * kpad : 0) + (int64_t)srow * kpad + sh;
 
-  for (int kc = 0; kc < kpad; kc += kKTile) {
-    // ---- stage the table tile
-    {
-      float v[16];
-      if (srow_ok) {
+  // table tile (16 floats of one row) -> registers
+  auto load_a = [&](int kc, float* v) {
+    if (srow_ok) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float4 t = *reinterpret_cast<const float4*>(Psrc + kc + 4 * q);
-          v[4 * q] = t.x;
-          v[4 * q + 1] = t.y;
-          v[4 * q + 2] = t.z;
-          v[4 * q + 3] = t.w;
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = 0.f;
+      for (int q = 0; q < 4; ++q) {
+        const float4 t = *reinterpret_cast<const float4*>(Psrc + kc + 4 * q);
+        v[4 * q] = t.x;
+        v[4 * q + 1] = t.y;
+        v[4 * q + 2] = t.z;
+        v[4 * q + 3] = t.w;
       }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = 0.f;
+    }
+  };
+  // registers -> LDS (split into BF16 hi / lo here for the BF16 paths)
+  auto store_a = [&](FwdLds<PREC>& L, const float* v) {
+    if constexpr (PREC == HPX_PREC_FP32) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) L.a[sp][sr][sh + q] = v[q];
+    } else if constexpr (PREC == HPX_PREC_BF16X3) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) split(v[q], L.hi[sp][sr][sh + q], L.lo[sp][sr][sh + q]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) L.hi[sp][sr][sh + q] = (__bf16)v[q];
+    }
+  };
+  // this wave's B operand (spectra) for one k chunk, in MFMA lane order
+  auto load_b = [&](int kc, float (&bv)[2][2][8]) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        const int col = col0 + ct * 16 + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          int kk;
+          if constexpr (PREC == HPX_PREC_FP32)
+            kk = kc + 4 * q + (lane >> 4);
+          else
+            kk = kc + 8 * (lane >> 4) + q;
+          bv[p][ct][q] = kk < npair ? Xm[p * pstride + (int64_t)kk * ncol + col] : 0.f;
+        }
+      }
+  };
+  auto compute = [&](const FwdLds<PREC>& L, const float (&bv)[2][2][8]) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
       if constexpr (PREC == HPX_PREC_FP32) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) lds.a[sp][sr][sh + q] = v[q];
-      } else if constexpr (PREC == HPX_PREC_BF16X3) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) split(v[q], lds.hi[sp][sr][sh + q], lds.lo[sp][sr][sh + q]);
-      } else {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) lds.hi[sp][sr][sh + q] = (__bf16)v[q];
-      }
-    }
-    // ---- this wave's B operand straight from the spectra
-    float bv[2][2][8];
-    if (wave_active) {
-#pragma unroll
-      for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
-          const int col = col0 + ct * 16 + (lane & 15);
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            int kk;
-            if constexpr (PREC == HPX_PREC_FP32)
-              kk = kc + 4 * q + (lane >> 4);
-            else
-              kk = kc + 8 * (lane >> 4) + q;
-            bv[p][ct][q] = kk < npair ? Xm[p * pstride + (int64_t)kk * ncol + col] : 0.f;
-          }
-        }
-    }
-    __syncthreads();
-    if (wave_active) {
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
           if (row0 + rt * 16 >= nlp[p]) continue;
           const int ar = rt * 16 + (lane & 15);
-          if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-              const float av = lds.a[p][ar][4 * q + (lane >> 4)];
+          for (int q = 0; q < 8; ++q) {
+            const float av = L.a[p][ar][4 * q + (lane >> 4)];
 #pragma unroll
-              for (int ct = 0; ct < 2; ++ct)
-                acc[p][rt][ct] =
-                    __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
-            }
-          } else {
-            const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&lds.hi[p][ar][8 * (lane >> 4)]);
-            bf16x8 alo;
-            if constexpr (PREC == HPX_PREC_BF16X3)
-              alo = *reinterpret_cast<const bf16x8*>(&lds.lo[p][ar][8 * (lane >> 4)]);
+            for (int ct = 0; ct < 2; ++ct)
+              acc[p][rt][ct] =
+                  __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
+          }
+        }
+      } else {
+        bf16x8 bhi[2], blo[2];
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
-              bf16x8 bhi, blo;
+        for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-              for (int q = 0; q < 8; ++q) {
-                __bf16 h, l;
-                split(bv[p][ct][q], h, l);
-                bhi[q] = h;
-                blo[q] = l;
-              }
-              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[p][rt][ct], 0, 0, 0);
-              if constexpr (PREC == HPX_PREC_BF16X3) {
-                acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc[p][rt][ct], 0, 0, 0);
-                acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc[p][rt][ct], 0, 0, 0);
-              }
+          for (int q = 0; q < 8; ++q) {
+            __bf16 h, l;
+            split(bv[p][ct][q], h, l);
+            bhi[ct][q] = h;
+            blo[ct][q] = l;
+          }
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          if (row0 + rt * 16 >= nlp[p]) continue;
+          const int ar = rt * 16 + (lane & 15);
+          const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&L.hi[p][ar][8 * (lane >> 4)]);
+          bf16x8 alo;
+          if constexpr (PREC == HPX_PREC_BF16X3)
+            alo = *reinterpret_cast<const bf16x8*>(&L.lo[p][ar][8 * (lane >> 4)]);
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct) {
+            acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi[ct], acc[p][rt][ct], 0, 0, 0);
+            if constexpr (PREC == HPX_PREC_BF16X3) {
+              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo[ct], acc[p][rt][ct], 0, 0, 0);
+              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi[ct], acc[p][rt][ct], 0, 0, 0);
             }
           }
         }
       }
+    }
+  };
+
+  // software pipeline: chunk c+1's loads are in flight while chunk c is
+  // multiplied; the table tile is double-buffered in LDS (one barrier per chunk)
+  const int nchunk = kpad / kKTile;
+  float pa[16];
+  float bv[2][2][8];
+  load_a(0, pa);
+  if (wave_active) load_b(0, bv);
+  store_a(lds[0], pa);
+  __syncthreads();
+  for (int c = 0; c < nchunk; ++c) {
+    const bool more = c + 1 < nchunk;
+    float pn[16];
+    float bn[2][2][8];
+    if (more) {
+      load_a((c + 1) * kKTile, pn);
+      if (wave_active) load_b((c + 1) * kKTile, bn);
+    }
+    if (wave_active) compute(lds[c & 1], bv);
+    if (more) {
+      store_a(lds[(c + 1) & 1], pn);
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) bv[p][ct][q] = bn[p][ct][q];
     }
     __syncthreads();
   }
@@ -341,7 +375,9 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
     const float* __restrict__ alm, float* __restrict__ X, const float* __restrict__ P,
     const int64_t* __restrict__ p_off, const int* __restrict__ nl, int nkb, int kpad, int lmax,
     int mmax, int64_t ncol, int64_t nbatch) {
-  __shared__ InvLds<PREC> lds;
+  __shared__ InvLds<PREC> lds[2];
+  // ring-pair block outer, m inner: consecutive m read the same 64-byte lines
+  // of a[b][l][m0..m0+8) and land on one XCD (contiguous item ranges)
   const int item = xcd_local_index(blockIdx.x, gridDim.x);
   const int kb = item / mmax;
   const int m = item - kb * mmax;
@@ -372,104 +408,136 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
   const int spad = sp ? padp1 : padp0;
   const float* Pbase = Pm + (sp ? (int64_t)padp0 * kpad : 0);
 
-  for (int ic = 0; ic < nmax; ic += kKTile) {
-    {
-      float v[16];
-      const int irow = ic + si;
-      const int kk = k0 + sq;
-      if (irow < spad && kk < kpad) {
-        const float* src = Pbase + (int64_t)irow * kpad + kk;
+  auto load_a = [&](int ic, float* v) {
+    const int irow = ic + si;
+    const int kk = k0 + sq;
+    if (irow < spad && kk < kpad) {
+      const float* src = Pbase + (int64_t)irow * kpad + kk;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
-          v[4 * q] = t.x;
-          v[4 * q + 1] = t.y;
-          v[4 * q + 2] = t.z;
-          v[4 * q + 3] = t.w;
-        }
-      } else {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) v[q] = 0.f;
+      for (int q = 0; q < 4; ++q) {
+        const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
+        v[4 * q] = t.x;
+        v[4 * q + 1] = t.y;
+        v[4 * q + 2] = t.z;
+        v[4 * q + 3] = t.w;
       }
-      if constexpr (PREC == HPX_PREC_FP32) {
+    } else {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) lds.a[sp][sq + q][si] = v[q];
-      } else if constexpr (PREC == HPX_PREC_BF16X3) {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) split(v[q], lds.hi[sp][sq + q][si], lds.lo[sp][sq + q][si]);
-      } else {
-#pragma unroll
-        for (int q = 0; q < 16; ++q) lds.hi[sp][sq + q][si] = (__bf16)v[q];
-      }
+      for (int q = 0; q < 16; ++q) v[q] = 0.f;
     }
-    float bv[2][2][8];
-    if (wave_active) {
+  };
+  auto store_a = [&](InvLds<PREC>& L, const float* v) {
+    if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
-      for (int p = 0; p < 2; ++p)
+      for (int q = 0; q < 16; ++q) L.a[sp][sq + q][si] = v[q];
+    } else if constexpr (PREC == HPX_PREC_BF16X3) {
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
-          const int col = col0 + ct * 16 + (lane & 15);
-          const int64_t b = col >> 1;
-          const int reim = col & 1;
+      for (int q = 0; q < 16; ++q) split(v[q], L.hi[sp][sq + q][si], L.lo[sp][sq + q][si]);
+    } else {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            int i;
-            if constexpr (PREC == HPX_PREC_FP32)
-              i = ic + 4 * q + (lane >> 4);
-            else
-              i = ic + 8 * (lane >> 4) + q;
-            float val = 0.f;
-            if (i < nlp[p] && b < nbatch) {
-              const int l = m + p + 2 * i;
-              val = alm[2 * ((b * lmax + l) * (int64_t)mmax + m) + reim];
-            }
-            bv[p][ct][q] = val;
+      for (int q = 0; q < 16; ++q) L.hi[sp][sq + q][si] = (__bf16)v[q];
+    }
+  };
+  auto load_b = [&](int ic, float (&bv)[2][2][8]) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        const int col = col0 + ct * 16 + (lane & 15);
+        const int64_t b = col >> 1;
+        const int reim = col & 1;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          int i;
+          if constexpr (PREC == HPX_PREC_FP32)
+            i = ic + 4 * q + (lane >> 4);
+          else
+            i = ic + 8 * (lane >> 4) + q;
+          float val = 0.f;
+          if (i < nlp[p] && b < nbatch) {
+            const int l = m + p + 2 * i;
+            val = alm[2 * ((b * lmax + l) * (int64_t)mmax + m) + reim];
           }
+          bv[p][ct][q] = val;
         }
-    }
-    __syncthreads();
-    if (wave_active) {
+      }
+  };
+  auto compute = [&](const InvLds<PREC>& L, const float (&bv)[2][2][8], int ic) {
 #pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        if (ic >= nlp[p]) continue;
+    for (int p = 0; p < 2; ++p) {
+      if (ic >= nlp[p]) continue;
+      if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
         for (int rt = 0; rt < 4; ++rt) {
           const int ar = rt * 16 + (lane & 15);
-          if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-              const float av = lds.a[p][ar][4 * q + (lane >> 4)];
+          for (int q = 0; q < 8; ++q) {
+            const float av = L.a[p][ar][4 * q + (lane >> 4)];
 #pragma unroll
-              for (int ct = 0; ct < 2; ++ct)
-                acc[p][rt][ct] =
-                    __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
-            }
-          } else {
-            const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&lds.hi[p][ar][8 * (lane >> 4)]);
-            bf16x8 alo;
-            if constexpr (PREC == HPX_PREC_BF16X3)
-              alo = *reinterpret_cast<const bf16x8*>(&lds.lo[p][ar][8 * (lane >> 4)]);
+            for (int ct = 0; ct < 2; ++ct)
+              acc[p][rt][ct] =
+                  __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
+          }
+        }
+      } else {
+        bf16x8 bhi[2], blo[2];
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct) {
-              bf16x8 bhi, blo;
+        for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-              for (int q = 0; q < 8; ++q) {
-                __bf16 h, l;
-                split(bv[p][ct][q], h, l);
-                bhi[q] = h;
-                blo[q] = l;
-              }
-              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi, acc[p][rt][ct], 0, 0, 0);
-              if constexpr (PREC == HPX_PREC_BF16X3) {
-                acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo, acc[p][rt][ct], 0, 0, 0);
-                acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi, acc[p][rt][ct], 0, 0, 0);
-              }
+          for (int q = 0; q < 8; ++q) {
+            __bf16 h, l;
+            split(bv[p][ct][q], h, l);
+            bhi[ct][q] = h;
+            blo[ct][q] = l;
+          }
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+          const int ar = rt * 16 + (lane & 15);
+          const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&L.hi[p][ar][8 * (lane >> 4)]);
+          bf16x8 alo;
+          if constexpr (PREC == HPX_PREC_BF16X3)
+            alo = *reinterpret_cast<const bf16x8*>(&L.lo[p][ar][8 * (lane >> 4)]);
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct) {
+            acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi[ct], acc[p][rt][ct], 0, 0, 0);
+            if constexpr (PREC == HPX_PREC_BF16X3) {
+              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo[ct], acc[p][rt][ct], 0, 0, 0);
+              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi[ct], acc[p][rt][ct], 0, 0, 0);
             }
           }
         }
       }
     }
+  };
+
+  const int nchunk = (nmax + kKTile - 1) / kKTile;
+  if (nchunk > 0) {
+    float pa[16];
+    float bv[2][2][8];
+    load_a(0, pa);
+    if (wave_active) load_b(0, bv);
+    store_a(lds[0], pa);
     __syncthreads();
+    for (int c = 0; c < nchunk; ++c) {
+      const bool more = c + 1 < nchunk;
+      float pn[16];
+      float bn[2][2][8];
+      if (more) {
+        load_a((c + 1) * kKTile, pn);
+        if (wave_active) load_b((c + 1) * kKTile, bn);
+      }
+      if (wave_active) compute(lds[c & 1], bv, c * kKTile);
+      if (more) {
+        store_a(lds[(c + 1) & 1], pn);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+          for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) bv[p][ct][q] = bn[p][ct][q];
+      }
+      __syncthreads();
+    }
   }
 
   if (!wave_active) return;

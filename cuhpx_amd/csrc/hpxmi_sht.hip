@@ -92,6 +92,7 @@ void free_plan(ShtPlan* p) {
   hipFree(p->d_p_off);
   hipFree(p->d_nl);
   hipFree(p->d_fwd_items);
+  hipFree(p->d_plist);
   delete p;
 }
 
@@ -167,6 +168,15 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   p->max_M = max_M;
   p->log2_max_M = ilog2(max_M);
 
+  // ring pairs grouped by FFT size class
+  std::vector<int> plist;
+  for (int c = 0; c < kFftClasses; ++c) {
+    p->class_off[c] = (int)plist.size();
+    for (int k = 0; k < p->npair; ++k)
+      if (fft_class_of(p->pairs[k].log2M) == c) plist.push_back(k);
+    p->class_n[c] = (int)plist.size() - p->class_off[c];
+  }
+
   std::vector<float2> tw(max_M);
   for (int x = 0; x < max_M; ++x) {
     const double ang = -2.0 * M_PI * (double)x / (double)max_M;
@@ -197,13 +207,15 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     const int nmax = std::max(p->nl[2 * m], p->nl[2 * m + 1]);
     max_blocks = std::max(max_blocks, (nmax + 63) / 64);
   }
-  // forward work list: row block outer, m inner (consecutive m share output
-  // cache lines and land on the same XCD)
-  for (int jb = 0; jb < max_blocks; ++jb)
-    for (int m = 0; m < mmax; ++m) {
-      const int nmax = std::max(p->nl[2 * m], p->nl[2 * m + 1]);
-      if (jb * 64 < nmax) items.push_back(make_int2(m, jb));
-    }
+  // forward work list: m outer, row block inner.  The work-groups of one m
+  // read the same spectra X[m] (512 KB at 64 fields) and consecutive m share
+  // output cache lines; the launch maps contiguous ranges of this list to one
+  // XCD, so both stay in that XCD's L2.
+  for (int m = 0; m < mmax; ++m) {
+    const int nmax = std::max(p->nl[2 * m], p->nl[2 * m + 1]);
+    for (int jb = 0; jb * 64 < nmax; ++jb) items.push_back(make_int2(m, jb));
+  }
+  (void)max_blocks;
   p->n_fwd_items = (int)items.size();
 
   std::vector<double> ct(p->kpad, 0.0);
@@ -218,7 +230,8 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
       (rc = upload(&p->d_twiddle, tw, bytes)) != HPX_OK ||
       (rc = upload(&p->d_p_off, p->p_off, bytes)) != HPX_OK ||
       (rc = upload(&p->d_nl, p->nl, bytes)) != HPX_OK ||
-      (rc = upload(&p->d_fwd_items, items, bytes)) != HPX_OK) {
+      (rc = upload(&p->d_fwd_items, items, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_plist, plist, bytes)) != HPX_OK) {
     free_plan(p);
     return rc;
   }

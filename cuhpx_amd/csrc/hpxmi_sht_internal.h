@@ -47,6 +47,9 @@ struct ShtPlan {
   int64_t* d_p_off = nullptr;   // [mmax + 1]
   int* d_nl = nullptr;          // [mmax * 2]
   int2* d_fwd_items = nullptr;  // forward GEMM work items (m, row block)
+  int* d_plist = nullptr;       // ring pairs grouped by FFT size class
+  int class_off[4] = {0, 0, 0, 0};
+  int class_n[4] = {0, 0, 0, 0};
   int n_fwd_items = 0;
   int64_t table_bytes = 0;
 };
@@ -59,8 +62,12 @@ constexpr int kKTile = 32;
 constexpr int kColBlock = 128;
 // Fields per FFT work-group.
 constexpr int kFieldsPerFftBlock = 8;
-// LDS capacity of the FFT kernels in complex values.
+// Largest FFT length (complex points) of the ring FFT kernels.
 constexpr int kFftLds = 8192;
+// FFT size classes: log2 M <= 10, 11, 12 (one wavefront per transform) and
+// 13 (one work-group per transform).
+constexpr int kFftClasses = 4;
+int fft_class_of(int log2M);
 
 inline int64_t col_pad(int64_t nbatch) { return ((2 * nbatch + 31) / 32) * 32; }
 
