@@ -44,6 +44,9 @@ def parse():
                     help="measure the pixel remaps at nside=1024, [64, npix] float32 instead")
     ap.add_argument("--calibrate", action="store_true",
                     help="measure the plain copy kernel (HBM yardstick) instead")
+    ap.add_argument("--mixed", action="store_true",
+                    help="measure the mixed-size request (32 fields each at nside 64, 128, 256, 512, "
+                         "lmax = mmax = 2 nside + 1) through one HIP graph launch instead")
     return ap.parse_args()
 
 
@@ -229,6 +232,72 @@ def bench_remaps(args):
                                    "unit": "GB/s", "frac": worst / PEAK["hbm_GBs"], "traffic": None}}))
 
 
+MIXED_NSIDES = (64, 128, 256, 512)
+MIXED_FIELDS = 32
+
+
+def bench_mixed(args):
+    """Ragged batch: 32 fields at each nside in {64, 128, 256, 512}, lmax =
+    mmax = 2 nside + 1, fwd + inv through one autotuned HIP graph launch."""
+    import cuhpx_amd
+    from oracle.sht import SHTOracle
+    nchk = 2
+    specs = [(n, MIXED_FIELDS, 2 * n + 1, 2 * n + 1) for n in MIXED_NSIDES]
+    # band-limited fields synthesised in FP64 (seed 10 + log2 nside); the
+    # first nchk are distinct and repeated to fill the batch
+    oracles, maps, alms, g_refs, a_refs = [], [], [], [], []
+    for n, b, L, _ in specs:
+        o = SHTOracle.healpix(n, L, L, streamed=n >= 512)
+        a = band_limited_alm(L, L, nchk, 10 + int(np.log2(n))).numpy()
+        # one FP64 sweep: the fields, and the inverse of their FP32-rounded coefficients
+        both = o.inverse(np.concatenate([a, a.astype(np.complex64).astype(np.complex128)]))
+        f64, g_ref = both[:nchk], both[nchk:]
+        x = np.tile(f64, (b // nchk, 1)).astype(np.float32)
+        a_refs.append(o.forward(x[:nchk].astype(np.float64)))
+        oracles.append(o)
+        alms.append(a)
+        g_refs.append(g_ref)
+        maps.append(torch.from_numpy(x))
+        print(f"[mixed] inputs and FP64 references nside={n} ready", file=sys.stderr, flush=True)
+    res = {}
+    for prec in ("bf16x3", "fp32"):
+        t0 = time.perf_counter()
+        mixed = cuhpx_amd.MixedSHT(specs, precision=prec, direction="both", autotune=True)
+        setup_s = time.perf_counter() - t0
+        for dst, x in zip(mixed.maps_in, maps):
+            dst.copy_(x)
+        ms = time_events(mixed.launch, args.steps, args.warmup)
+        acc = {}
+        for i, (n, b, L, _) in enumerate(specs):
+            a_ref = a_refs[i]
+            a = mixed.alm[i][:nchk].cpu().numpy()
+            ea = float(np.linalg.norm(a - a_ref) / np.linalg.norm(a_ref))
+            # inverse checked on the FP64 coefficients' FP32 rounding
+            isht = cuhpx_amd.iSHTCUDA(n, lmax=L, mmax=L, precision=prec)  # the branch's kernels and shapes
+            a32 = torch.from_numpy(alms[i].astype(np.complex64)).cuda()
+            g = isht(a32).cpu().numpy()
+            g_ref = g_refs[i]
+            ef = float(np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref))
+            acc[str(n)] = {"forward_rel_l2": ea, "inverse_rel_l2": ef}
+            print(f"[mixed {prec}] nside={n} forward {ea:.2e} inverse {ef:.2e}", file=sys.stderr, flush=True)
+        res[prec] = {"ms_per_launch": ms, "maps_per_s": len(specs) * MIXED_FIELDS / (ms * 1e-3),
+                     "setup_s": setup_s, "tuned_shapes": dict(zip(map(str, MIXED_NSIDES), mixed.tuned)),
+                     "accuracy_vs_fp64": acc}
+        del mixed
+    main = res["bf16x3"]
+    print(json.dumps({
+        "metric": "mixed-nside fwd+inv SHT maps/s, one HIP graph launch (32 fields each at nside 64/128/256/512)",
+        "value": main["maps_per_s"], "unit": "maps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": main["ms_per_launch"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (band-limited fields, C_l=(l+1)^-2, seeded; 2 distinct fields per size tiled to 32)",
+        "config": {"workload": "mixed nside {64,128,256,512} x 32 fields, lmax=mmax=2*nside+1, fwd+inv",
+                   "precision": "bf16x3", "parallelism": "single GPU"},
+        "fp32_maps_per_s": res["fp32"]["maps_per_s"],
+        "speedup_vs_fp32": res["fp32"]["ms_per_launch"] / main["ms_per_launch"],
+        "runs": res}))
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -247,6 +316,8 @@ def main():
         return bench_calibrate(args)
     if args.remaps:
         return bench_remaps(args)
+    if args.mixed:
+        return bench_mixed(args)
     t0 = time.perf_counter()
     plan = ShtPlan.healpix(NSIDE, LMAX, MMAX)
     torch.cuda.synchronize()

@@ -66,7 +66,19 @@ SIGNATURES = {
     "hpx_sht_stage_legendre_analysis": (_c_int, [_vp, _c_int, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_stage_legendre_synthesis": (_c_int, [_vp, _c_int, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_stage_ring_synthesis": (_c_int, [_vp, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_sht_autotune": (_c_int, [_vp, _c_int, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_sht_tuned_shapes": (_c_int, [_vp, _c_int, _c_i64, _i32p]),
+    "hpx_sht_mixed_workspace_bytes": (_c_i64, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64)]),
+    "hpx_sht_mixed_create": (_c_int, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), _c_int, _c_int,
+                                      ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
+                                      _c_i64, ctypes.POINTER(_vp)]),
+    "hpx_sht_mixed_launch": (_c_int, [_vp, _vp]),
+    "hpx_sht_mixed_destroy": (None, [_vp]),
 }
+
+MIXED_FORWARD = 1
+MIXED_INVERSE = 2
+MIXED_MAX_PLANS = 16
 
 
 def lib():

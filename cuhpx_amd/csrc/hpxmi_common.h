@@ -48,3 +48,7 @@ __device__ __forceinline__ int xcd_local_index(int bid, int n) {
   if ((n & 7) != 0) return bid;
   return (bid & 7) * (n >> 3) + (bid >> 3);
 }
+// Grid size for n work items launched with xcd_local_index(blockIdx.x,
+// gridDim.x): n rounded up to a multiple of 8, so that the remapping always
+// applies; work-groups whose item is >= n exit at once.
+inline unsigned xcd_grid(int64_t n) { return (unsigned)((n + 7) & ~int64_t(7)); }

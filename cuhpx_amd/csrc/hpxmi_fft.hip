@@ -668,43 +668,67 @@ StageArgs make_args(const ShtPlan& p, int64_t nbatch, int use_weight, float scal
   return a;
 }
 
+template <int WAVES, int C>
+void launch_wave(bool analysis, unsigned grid, size_t lds_bytes, const float* in, float* out,
+                 const StageArgs& a, hipStream_t s) {
+  constexpr int kLog2 = 10 + C;
+  if (analysis)
+    ring_analysis_wave_kernel<WAVES, (C == 0 ? 1 : kLog2), kLog2><<<grid, WAVES * 64, lds_bytes, s>>>(in, out, a);
+  else
+    ring_synthesis_wave_kernel<WAVES, (C == 0 ? 1 : kLog2), kLog2><<<grid, WAVES * 64, lds_bytes, s>>>(in, out, a);
+}
+
+}  // namespace
+
+int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, const float* in,
+                     float* out, int64_t nbatch, int use_weight, float scale, int cmode,
+                     hipStream_t s, int layout) {
+  const int n = p.class_n[c];
+  if (n == 0 || nbatch <= 0) return HPX_OK;
+  StageArgs a = make_args(p, nbatch, use_weight, scale, cmode, layout);
+  a.plist = p.d_plist + p.class_off[c];
+  if (c == kFftClasses - 1) {
+    const int64_t items = (int64_t)n * nbatch;
+    if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+    if (analysis)
+      ring_analysis_block_kernel<<<(unsigned)items, kBlockThreads, 0, s>>>(in, out, a);
+    else
+      ring_synthesis_block_kernel<<<(unsigned)items, kBlockThreads, 0, s>>>(in, out, a);
+    HPX_LAUNCH_CHECK();
+    return HPX_OK;
+  }
+  // class c < 3: FFT length <= 2^(10 + c); one field per wavefront
+  if (waves != kFftWaveChoices[c][0] && waves != kFftWaveChoices[c][1])
+    HPX_FAIL(HPX_ERR_ARG, "no FFT kernel with %d waves for size class %d", waves, c);
+  const int64_t nfg = (nbatch + waves - 1) / waves;
+  const int64_t items = (int64_t)n * nfg;
+  if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  const size_t lds_bytes = ((size_t)waves << (10 + c)) * sizeof(float2);
+  const unsigned grid = (unsigned)items;
+  switch (c * 16 + waves) {
+    case 0 * 16 + 4: launch_wave<4, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 0 * 16 + 8: launch_wave<8, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 1 * 16 + 2: launch_wave<2, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 1 * 16 + 4: launch_wave<4, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 2 * 16 + 1: launch_wave<1, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
+    default: launch_wave<2, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
+  }
+  HPX_LAUNCH_CHECK();
+  return HPX_OK;
+}
+
+namespace {
+
 template <bool ANALYSIS>
 int launch_stage(const ShtPlan& p, const float* in, float* out, int64_t nbatch, int use_weight,
                  float scale, int cmode, hipStream_t s, int layout) {
   if (nbatch <= 0) return HPX_OK;
+  const FftTune t = p.fft_tune(col_pad(nbatch));
   for (int c = 0; c < kFftClasses; ++c) {
-    const int n = p.class_n[c];
-    if (n == 0) continue;
-    StageArgs a = make_args(p, nbatch, use_weight, scale, cmode, layout);
-    a.plist = p.d_plist + p.class_off[c];
-    if (c == kFftClasses - 1) {
-      const int64_t items = (int64_t)n * nbatch;
-      if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-      if (ANALYSIS)
-        ring_analysis_block_kernel<<<(unsigned)items, kBlockThreads, 0, s>>>(in, out, a);
-      else
-        ring_synthesis_block_kernel<<<(unsigned)items, kBlockThreads, 0, s>>>(in, out, a);
-      HPX_LAUNCH_CHECK();
-      continue;
-    }
-    // class c < 3: FFT length <= 2^(10 + c); waves (= fields) per work-group
-    // chosen so that the work-group's LDS is 64 KB
-    const int waves = c == 0 ? 8 : (c == 1 ? 4 : 2);
-    const int64_t nfg = (nbatch + waves - 1) / waves;
-    const int64_t items = (int64_t)n * nfg;
-    if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-    const size_t lds_bytes = ((size_t)waves << (10 + c)) * sizeof(float2);
-    const unsigned grid = (unsigned)items;
-    if (ANALYSIS) {
-      if (c == 0) ring_analysis_wave_kernel<8, 1, 10><<<grid, 8 * 64, lds_bytes, s>>>(in, out, a);
-      else if (c == 1) ring_analysis_wave_kernel<4, 11, 11><<<grid, 4 * 64, lds_bytes, s>>>(in, out, a);
-      else ring_analysis_wave_kernel<2, 12, 12><<<grid, 2 * 64, lds_bytes, s>>>(in, out, a);
-    } else {
-      if (c == 0) ring_synthesis_wave_kernel<8, 1, 10><<<grid, 8 * 64, lds_bytes, s>>>(in, out, a);
-      else if (c == 1) ring_synthesis_wave_kernel<4, 11, 11><<<grid, 4 * 64, lds_bytes, s>>>(in, out, a);
-      else ring_synthesis_wave_kernel<2, 12, 12><<<grid, 2 * 64, lds_bytes, s>>>(in, out, a);
-    }
-    HPX_LAUNCH_CHECK();
+    const int waves = c < 3 ? t.waves[ANALYSIS ? 0 : 1][c] : 1;
+    const int rc = launch_fft_class(p, ANALYSIS, c, waves, in, out, nbatch, use_weight, scale, cmode, s,
+                                    layout);
+    if (rc != HPX_OK) return rc;
   }
   return HPX_OK;
 }

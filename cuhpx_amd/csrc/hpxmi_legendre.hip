@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void legendre_table_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Forward: a[b][l][m] = sum_k P[m][p][i][k] X[m][p][k][col]
+// Forward: aT[m][p][i][col] = sum_k P[m][p][i][k] X[m][p][k][col]
 // ---------------------------------------------------------------------------
 
 template <int PREC>
@@ -142,19 +142,19 @@ struct FwdLds<HPX_PREC_BF16> {
   __bf16 hi[2][kRowsPerParity][kKTile + kLdsPadH];
 };
 
-template <int PREC>
+template <int PREC, int CT>
 __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
-    const float* __restrict__ X, float* __restrict__ alm, const float* __restrict__ P,
+    const float* __restrict__ X, float* __restrict__ aT, const float* __restrict__ P,
     const int64_t* __restrict__ p_off, const int* __restrict__ nl,
-    const int2* __restrict__ items, int nitems, int npair, int kpad, int lmax, int mmax,
-    int64_t ncol, int64_t nbatch) {
+    const int2* __restrict__ items, int nitems, int npair, int kpad, int64_t ncol) {
   __shared__ FwdLds<PREC> lds[2];
-  const int item = xcd_local_index(blockIdx.x, nitems);
+  const int item = xcd_local_index(blockIdx.x, gridDim.x);
+  if (item >= nitems) return;
   const int m = items[item].x;
   const int jb = items[item].y;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int col0 = blockIdx.y * kColBlock + wave * 32;
+  const int col0 = blockIdx.y * (64 * CT) + wave * (16 * CT);
   const bool wave_active = col0 < ncol;
   const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
   const int padp0 = (nlp[0] + kRowPad - 1) / kRowPad * kRowPad;
@@ -164,13 +164,13 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
   const int64_t pstride = (int64_t)kpad * ncol;
   const float* Xm = X + (int64_t)m * 2 * pstride;
 
-  floatx4 acc[2][4][2];
+  floatx4 acc[2][4][CT];
 #pragma unroll
   for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) acc[p][rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int ct = 0; ct < CT; ++ct) acc[p][rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   // staging assignment: 2 parities x 64 rows x 2 halves of 16 k
   const int sp = threadIdx.x >> 7;
@@ -210,11 +210,11 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
     }
   };
   // this wave's B operand (spectra) for one k chunk, in MFMA lane order
-  auto load_b = [&](int kc, float (&bv)[2][2][8]) {
+  auto load_b = [&](int kc, float (&bv)[2][CT][8]) {
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
+      for (int ct = 0; ct < CT; ++ct) {
         const int col = col0 + ct * 16 + (lane & 15);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
         }
       }
   };
-  auto compute = [&](const FwdLds<PREC>& L, const float (&bv)[2][2][8]) {
+  auto compute = [&](const FwdLds<PREC>& L, const float (&bv)[2][CT][8]) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       if constexpr (PREC == HPX_PREC_FP32) {
@@ -239,15 +239,15 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
           for (int q = 0; q < 8; ++q) {
             const float av = L.a[p][ar][4 * q + (lane >> 4)];
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
+            for (int ct = 0; ct < CT; ++ct)
               acc[p][rt][ct] =
                   __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
           }
         }
       } else {
-        bf16x8 bhi[2], blo[2];
+        bf16x8 bhi[CT], blo[CT];
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
+        for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
           for (int q = 0; q < 8; ++q) {
             __bf16 h, l;
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
           if constexpr (PREC == HPX_PREC_BF16X3)
             alo = *reinterpret_cast<const bf16x8*>(&L.lo[p][ar][8 * (lane >> 4)]);
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct) {
+          for (int ct = 0; ct < CT; ++ct) {
             acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi[ct], acc[p][rt][ct], 0, 0, 0);
             if constexpr (PREC == HPX_PREC_BF16X3) {
               acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo[ct], acc[p][rt][ct], 0, 0, 0);
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
   // multiplied; the table tile is double-buffered in LDS (one barrier per chunk)
   const int nchunk = kpad / kKTile;
   float pa[16];
-  float bv[2][2][8];
+  float bv[2][CT][8];
   load_a(0, pa);
   if (wave_active) load_b(0, bv);
   store_a(lds[0], pa);
@@ -288,7 +288,7 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
   for (int c = 0; c < nchunk; ++c) {
     const bool more = c + 1 < nchunk;
     float pn[16];
-    float bn[2][2][8];
+    float bn[2][CT][8];
     if (more) {
       load_a((c + 1) * kKTile, pn);
       if (wave_active) load_b((c + 1) * kKTile, bn);
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
+        for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
           for (int q = 0; q < 8; ++q) bv[p][ct][q] = bn[p][ct][q];
     }
@@ -307,48 +307,25 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
   }
 
   if (!wave_active) return;
-  // ---- write a[b][l][m]; lanes 2c and 2c+1 hold re and im of one field.
-  const bool odd = lane & 1;
+  // ---- write this m's rows of aT: 16 lanes store 64 contiguous bytes of a row
+  float* Am = aT + (p_off[m] / kpad) * ncol;
 #pragma unroll
   for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
-        float mine[4], other[4];
+      for (int ct = 0; ct < CT; ++ct) {
+        const int col = col0 + ct * 16 + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          mine[r] = acc[p][rt][ct][r];
-          other[r] = __shfl_xor(mine[r], 1);
-        }
-        const int col = col0 + ct * 16 + (lane & 15);
-        const int64_t b = col >> 1;
-        if (b >= nbatch) continue;
-#pragma unroll
-        for (int rr = 0; rr < 2; ++rr) {
-          const int r = odd ? rr + 2 : rr;
-          const int i = row0 + rt * 16 + 4 * ((lane >> 4) & 3) + r;
-          if (i >= nlp[p]) continue;
-          const int l = m + p + 2 * i;
-          const float2 v = odd ? make_float2(other[r], mine[r]) : make_float2(mine[r], other[r]);
-          *reinterpret_cast<float2*>(alm + 2 * ((b * lmax + l) * (int64_t)mmax + m)) = v;
+          const int i = row0 + rt * 16 + 4 * (lane >> 4) + r;
+          if (i < nlp[p]) Am[(int64_t)((p ? padp0 : 0) + i) * ncol + col] = acc[p][rt][ct][r];
         }
       }
 }
 
-// a[b][l][m] = 0 for m > l (the reference's dense output has zeros there).
-__global__ __launch_bounds__(kThreads) void zero_upper_kernel(float* __restrict__ alm, int lmax,
-                                                              int mmax, int64_t nbatch) {
-  const int64_t row = blockIdx.x;  // (b, l)
-  const int l = (int)(row % lmax);
-  const int m0 = l + 1;
-  if (m0 >= mmax) return;
-  float2* dst = reinterpret_cast<float2*>(alm) + row * mmax;
-  for (int m = m0 + threadIdx.x; m < mmax; m += kThreads) dst[m] = make_float2(0.f, 0.f);
-}
-
 // ---------------------------------------------------------------------------
-// Inverse: Y[m][p][k][col] = sum_i P[m][p][i][k] a[b][m+p+2i][m]
+// Inverse: Y[m][p][k][col] = sum_i P[m][p][i][k] aT[m][p][i][col]
 // ---------------------------------------------------------------------------
 
 template <int PREC>
@@ -370,20 +347,21 @@ struct InvLds<HPX_PREC_BF16> {
   __bf16 hi[2][kKBlock][kKTile + kLdsPadH];
 };
 
-template <int PREC>
+template <int PREC, int CT>
 __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
-    const float* __restrict__ alm, float* __restrict__ X, const float* __restrict__ P,
-    const int64_t* __restrict__ p_off, const int* __restrict__ nl, int nkb, int kpad, int lmax,
-    int mmax, int64_t ncol, int64_t nbatch) {
+    const float* __restrict__ aT, float* __restrict__ X, const float* __restrict__ P,
+    const int64_t* __restrict__ p_off, const int* __restrict__ nl, int nkb, int nitems, int kpad,
+    int64_t ncol) {
   __shared__ InvLds<PREC> lds[2];
-  // ring-pair block outer, m inner: consecutive m read the same 64-byte lines
-  // of a[b][l][m0..m0+8) and land on one XCD (contiguous item ranges)
+  // m outer, ring-pair block inner: the nkb work-groups of one m read the same
+  // coefficient rows aT[m] and land on one XCD (contiguous item ranges)
   const int item = xcd_local_index(blockIdx.x, gridDim.x);
-  const int kb = item / mmax;
-  const int m = item - kb * mmax;
+  if (item >= nitems) return;
+  const int m = item / nkb;
+  const int kb = item - m * nkb;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int col0 = blockIdx.y * kColBlock + wave * 32;
+  const int col0 = blockIdx.y * (64 * CT) + wave * (16 * CT);
   const bool wave_active = col0 < ncol;
   const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
   const int padp0 = (nlp[0] + kRowPad - 1) / kRowPad * kRowPad;
@@ -393,13 +371,13 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
   const int64_t pstride = (int64_t)kpad * ncol;
   const int k0 = kb * kKBlock;
 
-  floatx4 acc[2][4][2];
+  floatx4 acc[2][4][CT];
 #pragma unroll
   for (int p = 0; p < 2; ++p)
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) acc[p][rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int ct = 0; ct < CT; ++ct) acc[p][rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
 
   // staging: 2 parities x 32 rows i x 4 quarters of 16 ring pairs
   const int sp = threadIdx.x >> 7;
@@ -438,14 +416,14 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
       for (int q = 0; q < 16; ++q) L.hi[sp][sq + q][si] = (__bf16)v[q];
     }
   };
-  auto load_b = [&](int ic, float (&bv)[2][2][8]) {
+  const float* Am = aT + (p_off[m] / kpad) * ncol;
+  auto load_b = [&](int ic, float (&bv)[2][CT][8]) {
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
+      for (int ct = 0; ct < CT; ++ct) {
         const int col = col0 + ct * 16 + (lane & 15);
-        const int64_t b = col >> 1;
-        const int reim = col & 1;
+        const float* src = Am + (int64_t)(p ? padp0 : 0) * ncol + col;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           int i;
@@ -453,16 +431,11 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
             i = ic + 4 * q + (lane >> 4);
           else
             i = ic + 8 * (lane >> 4) + q;
-          float val = 0.f;
-          if (i < nlp[p] && b < nbatch) {
-            const int l = m + p + 2 * i;
-            val = alm[2 * ((b * lmax + l) * (int64_t)mmax + m) + reim];
-          }
-          bv[p][ct][q] = val;
+          bv[p][ct][q] = i < nlp[p] ? src[(int64_t)i * ncol] : 0.f;
         }
       }
   };
-  auto compute = [&](const InvLds<PREC>& L, const float (&bv)[2][2][8], int ic) {
+  auto compute = [&](const InvLds<PREC>& L, const float (&bv)[2][CT][8], int ic) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
       if (ic >= nlp[p]) continue;
@@ -474,15 +447,15 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
           for (int q = 0; q < 8; ++q) {
             const float av = L.a[p][ar][4 * q + (lane >> 4)];
 #pragma unroll
-            for (int ct = 0; ct < 2; ++ct)
+            for (int ct = 0; ct < CT; ++ct)
               acc[p][rt][ct] =
                   __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
           }
         }
       } else {
-        bf16x8 bhi[2], blo[2];
+        bf16x8 bhi[CT], blo[CT];
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct)
+        for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
           for (int q = 0; q < 8; ++q) {
             __bf16 h, l;
@@ -498,7 +471,7 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
           if constexpr (PREC == HPX_PREC_BF16X3)
             alo = *reinterpret_cast<const bf16x8*>(&L.lo[p][ar][8 * (lane >> 4)]);
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct) {
+          for (int ct = 0; ct < CT; ++ct) {
             acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi[ct], acc[p][rt][ct], 0, 0, 0);
             if constexpr (PREC == HPX_PREC_BF16X3) {
               acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo[ct], acc[p][rt][ct], 0, 0, 0);
@@ -513,7 +486,7 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
   const int nchunk = (nmax + kKTile - 1) / kKTile;
   if (nchunk > 0) {
     float pa[16];
-    float bv[2][2][8];
+    float bv[2][CT][8];
     load_a(0, pa);
     if (wave_active) load_b(0, bv);
     store_a(lds[0], pa);
@@ -521,7 +494,7 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
     for (int c = 0; c < nchunk; ++c) {
       const bool more = c + 1 < nchunk;
       float pn[16];
-      float bn[2][2][8];
+      float bn[2][CT][8];
       if (more) {
         load_a((c + 1) * kKTile, pn);
         if (wave_active) load_b((c + 1) * kKTile, bn);
@@ -532,7 +505,7 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
 #pragma unroll
         for (int p = 0; p < 2; ++p)
 #pragma unroll
-          for (int ct = 0; ct < 2; ++ct)
+          for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
             for (int q = 0; q < 8; ++q) bv[p][ct][q] = bn[p][ct][q];
       }
@@ -547,7 +520,7 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
 #pragma unroll
     for (int rt = 0; rt < 4; ++rt)
 #pragma unroll
-      for (int ct = 0; ct < 2; ++ct) {
+      for (int ct = 0; ct < CT; ++ct) {
         const int col = col0 + ct * 16 + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -555,6 +528,106 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
           if (k < kpad) Xm[p * pstride + (int64_t)k * ncol + col] = acc[p][rt][ct][r];
         }
       }
+}
+
+// ---------------------------------------------------------------------------
+// Coefficient layout changes between aT (GEMM order, one row of fields per
+// (m, l)) and the user layout a[b][l][m] (complex, m fastest).  A tile is one
+// l, 32 consecutive m and 64 fields: 32 rows of 512 bytes on the aT side and
+// 64 runs of 256 bytes on the user side, transposed through LDS.
+// ---------------------------------------------------------------------------
+
+constexpr int kTrM = 32;        // m per tile
+constexpr int kTrCols = 128;    // aT columns per tile (64 fields, re / im)
+
+__device__ __forceinline__ int64_t at_row(const int64_t* p_off, const int* nl, int kpad, int m,
+                                          int l) {
+  const int d = l - m;
+  const int pad0 = (nl[2 * m] + kRowPad - 1) / kRowPad * kRowPad;
+  return p_off[m] / kpad + ((d & 1) ? pad0 : 0) + (d >> 1);
+}
+
+// a[b][l][m] from aT; zero for l < m < mmax (the reference's dense layout).
+__global__ __launch_bounds__(kThreads) void coeff_to_user_kernel(
+    const float* __restrict__ aT, float* __restrict__ alm, const int64_t* __restrict__ p_off,
+    const int* __restrict__ nl, int kpad, int lmax, int mmax, int64_t ncol, int64_t nbatch) {
+  __shared__ float t[kTrM][kTrCols + 1];
+  const int l = blockIdx.x;
+  const int m0 = blockIdx.y * kTrM;
+  const int64_t c0 = (int64_t)blockIdx.z * kTrCols;
+  const int mend = min(mmax, l + 1);
+  const int tid = threadIdx.x;
+  {
+    const int r = tid >> 3;             // tile row (m - m0)
+    const int cq = (tid & 7) * 4;       // float4 within a 32-float run
+    const int m = m0 + r;
+    if (m < mend) {
+      const float* src = aT + at_row(p_off, nl, kpad, m, l) * ncol + c0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int c = cq + 32 * j;
+        if (c0 + c < ncol) {
+          const float4 v = *reinterpret_cast<const float4*>(src + c);
+          t[r][c] = v.x;
+          t[r][c + 1] = v.y;
+          t[r][c + 2] = v.z;
+          t[r][c + 3] = v.w;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const int mm = tid & 31;
+  const int m = m0 + mm;
+  if (m >= mmax) return;
+  const bool zero = m >= mend;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int bs = (tid >> 5) + 8 * j;  // field within the tile
+    const int64_t b = c0 / 2 + bs;
+    if (b >= nbatch) break;
+    const float2 v = zero ? make_float2(0.f, 0.f) : make_float2(t[mm][2 * bs], t[mm][2 * bs + 1]);
+    *reinterpret_cast<float2*>(alm + 2 * ((b * lmax + l) * (int64_t)mmax + m)) = v;
+  }
+}
+
+// aT from a[b][l][m] (rows m <= l only); columns of fields >= nbatch are zero.
+__global__ __launch_bounds__(kThreads) void user_to_coeff_kernel(
+    const float* __restrict__ alm, float* __restrict__ aT, const int64_t* __restrict__ p_off,
+    const int* __restrict__ nl, int kpad, int lmax, int mmax, int64_t ncol, int64_t nbatch) {
+  __shared__ float t[kTrM][kTrCols + 1];
+  const int l = blockIdx.x;
+  const int m0 = blockIdx.y * kTrM;
+  const int64_t c0 = (int64_t)blockIdx.z * kTrCols;
+  const int mend = min(mmax, l + 1);
+  if (m0 >= mend) return;
+  const int tid = threadIdx.x;
+  {
+    const int mm = tid & 31;
+    const int m = m0 + mm;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int bs = (tid >> 5) + 8 * j;
+      const int64_t b = c0 / 2 + bs;
+      float2 v = make_float2(0.f, 0.f);
+      if (m < mend && b < nbatch)
+        v = *reinterpret_cast<const float2*>(alm + 2 * ((b * lmax + l) * (int64_t)mmax + m));
+      t[mm][2 * bs] = v.x;
+      t[mm][2 * bs + 1] = v.y;
+    }
+  }
+  __syncthreads();
+  const int r = tid >> 3;
+  const int cq = (tid & 7) * 4;
+  const int m = m0 + r;
+  if (m >= mend) return;
+  float* dst = aT + at_row(p_off, nl, kpad, m, l) * ncol + c0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int c = cq + 32 * j;
+    if (c0 + c < ncol)
+      *reinterpret_cast<float4*>(dst + c) = make_float4(t[r][c], t[r][c + 1], t[r][c + 2], t[r][c + 3]);
+  }
 }
 
 }  // namespace
@@ -567,68 +640,103 @@ int launch_legendre_table(ShtPlan& p, const double* d_ct, hipStream_t s) {
   return HPX_OK;
 }
 
-int launch_legendre_fwd(const ShtPlan& p, int precision, const float* X, float* alm,
-                        int64_t nbatch, hipStream_t s) {
-  if (nbatch <= 0) return HPX_OK;
+static dim3 transpose_grid(const ShtPlan& p, int64_t ncol) {
+  return dim3((unsigned)p.lmax, (unsigned)((p.mmax + kTrM - 1) / kTrM),
+              (unsigned)((ncol + kTrCols - 1) / kTrCols));
+}
+
+int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, float* aT,
+                    int64_t nbatch, hipStream_t s) {
+  if (nbatch <= 0 || p.n_fwd_items == 0) return HPX_OK;
   const int64_t ncol = col_pad(nbatch);
-  const int64_t ycols = (ncol + kColBlock - 1) / kColBlock;
+  const int64_t ycols = (ncol + 64 * ct - 1) / (64 * ct);
   if (ycols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  if (p.n_fwd_items > 0) {
-    dim3 grid(p.n_fwd_items, (unsigned)ycols);
-    switch (precision) {
-      case HPX_PREC_FP32:
-        legendre_fwd_kernel<HPX_PREC_FP32><<<grid, kThreads, 0, s>>>(
-            X, alm, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items, p.n_fwd_items, p.npair, p.kpad,
-            p.lmax, p.mmax, ncol, nbatch);
-        break;
-      case HPX_PREC_BF16X3:
-        legendre_fwd_kernel<HPX_PREC_BF16X3><<<grid, kThreads, 0, s>>>(
-            X, alm, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items, p.n_fwd_items, p.npair, p.kpad,
-            p.lmax, p.mmax, ncol, nbatch);
-        break;
-      case HPX_PREC_BF16:
-        legendre_fwd_kernel<HPX_PREC_BF16><<<grid, kThreads, 0, s>>>(
-            X, alm, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items, p.n_fwd_items, p.npair, p.kpad,
-            p.lmax, p.mmax, ncol, nbatch);
-        break;
-      default:
-        HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
-    }
-    HPX_LAUNCH_CHECK();
+  const dim3 grid(xcd_grid(p.n_fwd_items), (unsigned)ycols);
+#define HPX_FWD(PR, CT)                                                                      \
+  legendre_fwd_kernel<PR, CT><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl,     \
+                                                        p.d_fwd_items, p.n_fwd_items, p.npair, \
+                                                        p.kpad, ncol)
+  switch (precision * 4 + ct) {
+    case HPX_PREC_FP32 * 4 + 1: HPX_FWD(HPX_PREC_FP32, 1); break;
+    case HPX_PREC_FP32 * 4 + 2: HPX_FWD(HPX_PREC_FP32, 2); break;
+    case HPX_PREC_BF16X3 * 4 + 1: HPX_FWD(HPX_PREC_BF16X3, 1); break;
+    case HPX_PREC_BF16X3 * 4 + 2: HPX_FWD(HPX_PREC_BF16X3, 2); break;
+    case HPX_PREC_BF16 * 4 + 1: HPX_FWD(HPX_PREC_BF16, 1); break;
+    case HPX_PREC_BF16 * 4 + 2: HPX_FWD(HPX_PREC_BF16, 2); break;
+    default: HPX_FAIL(HPX_ERR_ARG, "no forward GEMM for precision %d, %d column tiles", precision, ct);
   }
-  if (p.mmax > 1 && p.lmax > 0) {
-    zero_upper_kernel<<<(unsigned)(nbatch * p.lmax), kThreads, 0, s>>>(alm, p.lmax, p.mmax, nbatch);
-    HPX_LAUNCH_CHECK();
-  }
+#undef HPX_FWD
+  HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
 
-int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float* X,
-                        int64_t nbatch, hipStream_t s) {
+int launch_gemm_inv(const ShtPlan& p, int precision, int ct, const float* aT, float* X,
+                    int64_t nbatch, hipStream_t s) {
   if (nbatch <= 0) return HPX_OK;
   const int64_t ncol = col_pad(nbatch);
-  const int64_t ycols = (ncol + kColBlock - 1) / kColBlock;
+  const int64_t ycols = (ncol + 64 * ct - 1) / (64 * ct);
   if (ycols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   const int nkb = (p.kpad + kKBlock - 1) / kKBlock;
-  dim3 grid((unsigned)(nkb * p.mmax), (unsigned)ycols);
-  switch (precision) {
-    case HPX_PREC_FP32:
-      legendre_inv_kernel<HPX_PREC_FP32><<<grid, kThreads, 0, s>>>(
-          alm, X, p.d_p, p.d_p_off, p.d_nl, nkb, p.kpad, p.lmax, p.mmax, ncol, nbatch);
-      break;
-    case HPX_PREC_BF16X3:
-      legendre_inv_kernel<HPX_PREC_BF16X3><<<grid, kThreads, 0, s>>>(
-          alm, X, p.d_p, p.d_p_off, p.d_nl, nkb, p.kpad, p.lmax, p.mmax, ncol, nbatch);
-      break;
-    case HPX_PREC_BF16:
-      legendre_inv_kernel<HPX_PREC_BF16><<<grid, kThreads, 0, s>>>(
-          alm, X, p.d_p, p.d_p_off, p.d_nl, nkb, p.kpad, p.lmax, p.mmax, ncol, nbatch);
-      break;
-    default:
-      HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
+  const int nitems = nkb * p.mmax;
+  const dim3 grid(xcd_grid(nitems), (unsigned)ycols);
+#define HPX_INV(PR, CT)                                                                           \
+  legendre_inv_kernel<PR, CT><<<grid, kThreads, 0, s>>>(aT, X, p.d_p, p.d_p_off, p.d_nl, nkb, nitems, \
+                                                        p.kpad, ncol)
+  switch (precision * 4 + ct) {
+    case HPX_PREC_FP32 * 4 + 1: HPX_INV(HPX_PREC_FP32, 1); break;
+    case HPX_PREC_FP32 * 4 + 2: HPX_INV(HPX_PREC_FP32, 2); break;
+    case HPX_PREC_BF16X3 * 4 + 1: HPX_INV(HPX_PREC_BF16X3, 1); break;
+    case HPX_PREC_BF16X3 * 4 + 2: HPX_INV(HPX_PREC_BF16X3, 2); break;
+    case HPX_PREC_BF16 * 4 + 1: HPX_INV(HPX_PREC_BF16, 1); break;
+    case HPX_PREC_BF16 * 4 + 2: HPX_INV(HPX_PREC_BF16, 2); break;
+    default: HPX_FAIL(HPX_ERR_ARG, "no inverse GEMM for precision %d, %d column tiles", precision, ct);
   }
+#undef HPX_INV
   HPX_LAUNCH_CHECK();
   return HPX_OK;
+}
+
+int launch_coeff_to_user(const ShtPlan& p, const float* aT, float* alm, int64_t nbatch,
+                         hipStream_t s) {
+  if (nbatch <= 0) return HPX_OK;
+  const int64_t ncol = col_pad(nbatch);
+  if ((ncol + kTrCols - 1) / kTrCols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  coeff_to_user_kernel<<<transpose_grid(p, ncol), kThreads, 0, s>>>(aT, alm, p.d_p_off, p.d_nl, p.kpad,
+                                                                   p.lmax, p.mmax, ncol, nbatch);
+  HPX_LAUNCH_CHECK();
+  return HPX_OK;
+}
+
+int launch_user_to_coeff(const ShtPlan& p, const float* alm, float* aT, int64_t nbatch,
+                         hipStream_t s) {
+  if (nbatch <= 0) return HPX_OK;
+  const int64_t ncol = col_pad(nbatch);
+  if ((ncol + kTrCols - 1) / kTrCols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  user_to_coeff_kernel<<<transpose_grid(p, ncol), kThreads, 0, s>>>(alm, aT, p.d_p_off, p.d_nl, p.kpad,
+                                                                   p.lmax, p.mmax, ncol, nbatch);
+  HPX_LAUNCH_CHECK();
+  return HPX_OK;
+}
+
+int launch_legendre_fwd(const ShtPlan& p, int precision, float* ws, float* alm,
+                        int64_t nbatch, hipStream_t s) {
+  if (nbatch <= 0) return HPX_OK;
+  float* aT = ws + x_floats(p, nbatch);
+  const GemmTune t = p.gemm_tune(precision, col_pad(nbatch));
+  int rc = launch_gemm_fwd(p, precision, t.fwd_ct, ws, aT, nbatch, s);
+  if (rc != HPX_OK) return rc;
+  return launch_coeff_to_user(p, aT, alm, nbatch, s);
+}
+
+int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float* ws,
+                        int64_t nbatch, hipStream_t s) {
+  if (nbatch <= 0) return HPX_OK;
+  if (precision < 0 || precision > 2) HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
+  float* aT = ws + x_floats(p, nbatch);
+  const GemmTune t = p.gemm_tune(precision, col_pad(nbatch));
+  int rc = launch_user_to_coeff(p, alm, aT, nbatch, s);
+  if (rc != HPX_OK) return rc;
+  return launch_gemm_inv(p, precision, t.inv_ct, aT, ws, nbatch, s);
 }
 
 }  // namespace hpxmi

@@ -244,6 +244,7 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     }
   }
   const int64_t pfloats = p->p_off[mmax];
+  p->at_rows = pfloats / p->kpad;
   double* d_ct = nullptr;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_p), std::max<int64_t>(pfloats, 1) * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_ct), ct.size() * sizeof(double));
@@ -428,7 +429,7 @@ extern "C" int hpx_sht_stage_legendre_analysis(const hpx_sht_plan* plan, int pre
   const ShtPlan* pl = P(plan);
   int rc = check_call(pl, precision, nbatch, workspace_bytes);
   if (rc != HPX_OK || nbatch == 0) return rc;
-  return launch_legendre_fwd(*pl, precision, static_cast<const float*>(workspace), alm, nbatch, stream);
+  return launch_legendre_fwd(*pl, precision, static_cast<float*>(workspace), alm, nbatch, stream);
 }
 
 extern "C" int hpx_sht_stage_legendre_synthesis(const hpx_sht_plan* plan, int precision,

@@ -3,9 +3,28 @@
 
 #include "hpxmi_common.h"
 
+#include <map>
+#include <mutex>
+#include <utility>
 #include <vector>
 
 namespace hpxmi {
+
+// Launch shapes of the ring FFT stage for one column count: fields (one per
+// wavefront) per work-group for the wave-level size classes 0..2, for the
+// analysis [0] and synthesis [1] directions.
+struct FftTune {
+  int waves[2][3] = {{8, 4, 2}, {8, 4, 2}};
+};
+// Launch shapes of the Legendre GEMMs for one (precision, column count):
+// 16-column MFMA tiles per wavefront (work-group column block = 64 * ct).
+struct GemmTune {
+  int fwd_ct = 2;
+  int inv_ct = 2;
+};
+// Allowed values (what the kernels are instantiated for).
+constexpr int kFftWaveChoices[3][2] = {{4, 8}, {2, 4}, {1, 2}};
+constexpr int kGemmCtChoices[2] = {1, 2};
 
 // One mirror pair of rings (north t, south nring-1-t); the equator of a grid
 // with an odd number of rings is a pair with north == south (single = 1).
@@ -51,15 +70,33 @@ struct ShtPlan {
   int class_off[4] = {0, 0, 0, 0};
   int class_n[4] = {0, 0, 0, 0};
   int n_fwd_items = 0;
+  int64_t at_rows = 0;  // rows of the P table = rows of the coefficient workspace aT
   int64_t table_bytes = 0;
+
+  // Autotuned launch shapes (hpx_sht_autotune); untuned keys use defaults.
+  mutable std::mutex tune_mu;
+  std::map<int64_t, FftTune> fft_tunes;                   // by column count
+  std::map<std::pair<int, int64_t>, GemmTune> gemm_tunes;  // by (precision, column count)
+
+  FftTune fft_tune(int64_t ncol) const {
+    std::lock_guard<std::mutex> g(tune_mu);
+    auto it = fft_tunes.find(ncol);
+    return it == fft_tunes.end() ? FftTune{} : it->second;
+  }
+  GemmTune gemm_tune(int precision, int64_t ncol) const {
+    std::lock_guard<std::mutex> g(tune_mu);
+    auto it = gemm_tunes.find({precision, ncol});
+    if (it != gemm_tunes.end()) return it->second;
+    GemmTune t;
+    if (ncol <= 64) t.fwd_ct = t.inv_ct = 1;  // one column block keeps all waves busy
+    return t;
+  }
 };
 
 // Rows of the P table per (m, parity) are padded to this many.
 constexpr int kRowPad = 16;
 // Ring-pair (K) tile of the Legendre GEMMs.
 constexpr int kKTile = 32;
-// Columns (2 * fields: re, im) per Legendre work-group.
-constexpr int kColBlock = 128;
 // Fields per FFT work-group.
 constexpr int kFieldsPerFftBlock = 8;
 // Largest FFT length (complex points) of the ring FFT kernels.
@@ -71,9 +108,14 @@ int fft_class_of(int log2M);
 
 inline int64_t col_pad(int64_t nbatch) { return ((2 * nbatch + 31) / 32) * 32; }
 
-// Workspace: X[mmax][2][kpad][ncol_pad] float (ring-pair spectra, S and D).
-inline int64_t workspace_floats(const ShtPlan& p, int64_t nbatch) {
+// Workspace: X[mmax][2][kpad][ncol] float (ring-pair spectra, S and D),
+// then aT[at_rows][ncol] float: the coefficients in P-table row order
+// (m, parity, row i with l = m + parity + 2 i), fields (re, im) along the row.
+inline int64_t x_floats(const ShtPlan& p, int64_t nbatch) {
   return (int64_t)p.mmax * 2 * p.kpad * col_pad(nbatch);
+}
+inline int64_t workspace_floats(const ShtPlan& p, int64_t nbatch) {
+  return x_floats(p, nbatch) + p.at_rows * col_pad(nbatch);
 }
 
 // Launchers (return HPX_OK or an error code after setting the error text).
@@ -83,9 +125,24 @@ int launch_analysis(const ShtPlan& p, const float* f, float* X, int64_t nbatch,
                     int use_weight, float scale, int cmode, hipStream_t s, int layout = 0);
 int launch_synthesis(const ShtPlan& p, const float* X, float* f, int64_t nbatch,
                      int use_weight, float scale, int cmode, hipStream_t s, int layout = 0);
-int launch_legendre_fwd(const ShtPlan& p, int precision, const float* X, float* alm,
+// One size class of the ring FFT stage with an explicit work-group shape.
+int launch_fft_class(const ShtPlan& p, bool analysis, int cls, int waves, const float* in,
+                     float* out, int64_t nbatch, int use_weight, float scale, int cmode,
+                     hipStream_t s, int layout);
+// The GEMMs alone (X <-> aT) with an explicit column tiling, and the
+// coefficient layout changes aT <-> a[b][l][m].
+int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, float* aT,
+                    int64_t nbatch, hipStream_t s);
+int launch_gemm_inv(const ShtPlan& p, int precision, int ct, const float* aT, float* X,
+                    int64_t nbatch, hipStream_t s);
+int launch_coeff_to_user(const ShtPlan& p, const float* aT, float* alm, int64_t nbatch,
+                         hipStream_t s);
+int launch_user_to_coeff(const ShtPlan& p, const float* alm, float* aT, int64_t nbatch,
+                         hipStream_t s);
+// The Legendre launchers take the whole workspace (X, then aT).
+int launch_legendre_fwd(const ShtPlan& p, int precision, float* ws, float* alm,
                         int64_t nbatch, hipStream_t s);
-int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float* X,
+int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float* ws,
                         int64_t nbatch, hipStream_t s);
 int launch_legendre_table(ShtPlan& p, const double* theta_north, hipStream_t s);
 

@@ -115,6 +115,25 @@ class ShtPlan:
         nbytes = int(_lib.lib().hpx_sht_workspace_bytes(self._handle, nbatch))
         return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device), nbytes
 
+    def autotune(self, precision, nbatch):
+        """Measure and keep the fastest FFT / GEMM launch shapes for this
+        precision and batch (synchronises the current stream)."""
+        ws, nbytes = self.workspace(nbatch)
+        with torch.cuda.device(self.device):
+            rc = _lib.lib().hpx_sht_autotune(self._handle, _lib.PRECISIONS[precision], nbatch, _lib.ptr(ws),
+                                             nbytes, _lib.stream_ptr(self.device))
+        _lib.check(rc, "SHT autotune")
+        return self.tuned_shapes(precision, nbatch)
+
+    def tuned_shapes(self, precision, nbatch):
+        """Launch shapes used for (precision, nbatch)."""
+        out = (ctypes.c_int32 * 8)()
+        rc = _lib.lib().hpx_sht_tuned_shapes(self._handle, _lib.PRECISIONS[precision], nbatch, out)
+        _lib.check(rc, "SHT tuned shapes")
+        v = list(out)
+        return {"fft_analysis_waves": v[0:3], "fft_synthesis_waves": v[3:6], "gemm_fwd_col_tiles": v[6],
+                "gemm_inv_col_tiles": v[7]}
+
     def _run(self, fn, *args, nbatch):
         ws, nbytes = self.workspace(nbatch)
         with torch.cuda.device(self.device):

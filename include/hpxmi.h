@@ -5,7 +5,8 @@
  * allocated by the caller.  Every call is asynchronous on the given stream
  * and returns HPX_OK or an error code; hpx_last_error() then describes the
  * failure (thread-local).  No call allocates device memory except plan
- * creation.
+ * creation, hpx_sht_autotune (temporary buffers, freed before it returns)
+ * and hpx_sht_mixed_create (the graph it instantiates).
  *
  * Layouts (row-major, last index fastest):
  *   map  f    float32 [nbatch, npix]                 (RING order for HEALPix)
@@ -180,10 +181,11 @@ int hpx_ring_irfft(const hpx_sht_plan* plan, const float* ftm, float* f, int64_t
                    hipStream_t stream);
 
 /* Split-phase entry points: the two stages of each direction, with the
- * ring-pair spectra left in the workspace between them (layout
+ * ring-pair spectra left at the start of the workspace between them (layout
  * [mmax][2][kpad][ncol] float32, kpad = ring pairs rounded up to 32,
  * ncol = 2*nbatch rounded up to 32; parity 0 holds S = w (F_t + F_t'),
- * parity 1 holds D = w (F_t - F_t')).  hpx_sht_forward is
+ * parity 1 holds D = w (F_t - F_t')); the Legendre stages also use the rest
+ * of the workspace for the coefficients in GEMM order.  hpx_sht_forward is
  * hpx_sht_stage_ring_analysis followed by hpx_sht_stage_legendre_analysis;
  * hpx_sht_inverse is hpx_sht_stage_legendre_synthesis followed by
  * hpx_sht_stage_ring_synthesis.  No counterpart in the reference (its stages
@@ -199,6 +201,51 @@ int hpx_sht_stage_legendre_synthesis(const hpx_sht_plan* plan, int precision, co
                                      hipStream_t stream);
 int hpx_sht_stage_ring_synthesis(const hpx_sht_plan* plan, float* f, int64_t nbatch,
                                  void* workspace, int64_t workspace_bytes, hipStream_t stream);
+
+/* ------------------------------------------------------------------ */
+/* Launch-shape autotuning and mixed-size batched requests             */
+/* ------------------------------------------------------------------ */
+
+/* Measures the launch shapes of the ring FFT stage (fields per work-group of
+ * each FFT size class, both directions) and of the two Legendre GEMMs
+ * (column tiles per wavefront) for this plan, precision and batch on the
+ * stream's device, and keeps the fastest in the plan; later calls with the
+ * same precision and batch column count use them.  Synchronises the stream;
+ * must not be called while the stream is being captured.  No counterpart in
+ * the reference (its launch shapes are fixed, hpx_fft_cuda.cu:1062-1332). */
+int hpx_sht_autotune(hpx_sht_plan* plan, int precision, int64_t nbatch, void* workspace,
+                     int64_t workspace_bytes, hipStream_t stream);
+
+/* The launch shapes used for (precision, nbatch): shapes[0..2] analysis
+ * fields per work-group of FFT classes 0..2, shapes[3..5] the same for
+ * synthesis, shapes[6] / shapes[7] forward / inverse GEMM column tiles. */
+int hpx_sht_tuned_shapes(const hpx_sht_plan* plan, int precision, int64_t nbatch,
+                         int32_t* shapes);
+
+/* A mixed-size request: nplans HEALPix (or ring-grid) plans, each with its
+ * own batch of maps, run as one HIP graph whose branches (one per plan) run
+ * concurrently.  direction HPX_MIXED_FORWARD: maps_in[i] -> alm[i];
+ * HPX_MIXED_INVERSE: alm[i] -> maps_out[i]; both: maps_in[i] -> alm[i] ->
+ * maps_out[i].  The buffers and the workspace are bound at creation (static
+ * buffers): each hpx_sht_mixed_launch reads and writes the same addresses.
+ * Replaces the reference's per-size SHTCUDA / iSHTCUDA calls (hpx_sht.py:
+ * 685-691, 732-735), whose FFT plan cache is rebuilt whenever nside changes
+ * (hpx_fft.cpp:49-58).  Launches that overlap on the same request race. */
+typedef struct hpx_sht_mixed hpx_sht_mixed;
+#define HPX_MIXED_FORWARD 1
+#define HPX_MIXED_INVERSE 2
+#define HPX_MIXED_MAX_PLANS 16
+
+/* Workspace bytes for a mixed request (sum of the per-plan workspaces,
+ * each rounded up to 256 bytes); -1 on a bad argument. */
+int64_t hpx_sht_mixed_workspace_bytes(int nplans, const hpx_sht_plan* const* plans,
+                                      const int64_t* nbatch);
+int hpx_sht_mixed_create(int nplans, const hpx_sht_plan* const* plans, const int64_t* nbatch,
+                         int precision, int direction, const float* const* maps_in,
+                         float* const* alm, float* const* maps_out, void* workspace,
+                         int64_t workspace_bytes, hpx_sht_mixed** mixed);
+int hpx_sht_mixed_launch(hpx_sht_mixed* mixed, hipStream_t stream);
+void hpx_sht_mixed_destroy(hpx_sht_mixed* mixed);
 
 #ifdef __cplusplus
 }

@@ -61,6 +61,61 @@ _BIG = 2.0 ** _SCALE_BITS
 _TINY = 2.0 ** -_SCALE_BITS
 
 
+def legpoly_scaled_lrows(mmax, lmax, x):
+    """Generator of (l, Q) with Q[m, k] = P_lm(x_k) for m < min(l + 1, mmax):
+    the values of legpoly_scaled with l as the outer loop and all m advanced
+    together (large vectors, one l-row in memory).  Same scaled recurrence:
+    the sectoral start P_ll is the running product over l, each value carried
+    as v * 2**(600 e) and emitted when e == 0."""
+    x = np.asarray(x, dtype=np.float64)
+    nx = len(x)
+    s = np.sqrt((1 + x) * (1 - x))
+    pmm = np.full(nx, 1.0 / np.sqrt(4 * np.pi))
+    emm = np.zeros(nx, dtype=np.int64)
+    cur = np.zeros((mmax, nx))                 # P_{l-1, m} (scaled)
+    prev = np.zeros((mmax, nx))                # P_{l-2, m} (scaled)
+    e = np.zeros((mmax, nx), dtype=np.int64)   # exponents of cur / prev
+    scaled = np.zeros(mmax, dtype=bool)        # rows holding a negative exponent
+
+    def rescale(r0, r1):
+        rows = r0 + np.flatnonzero(scaled[r0:r1])
+        if len(rows) == 0:
+            return
+        c, p, ee = cur[rows], prev[rows], e[rows]
+        big = (ee < 0) & (np.abs(c) > 1.0)
+        cur[rows] = np.where(big, c * _TINY, c)
+        prev[rows] = np.where(big, p * _TINY, p)
+        ee += big.astype(np.int64)
+        e[rows] = ee
+        scaled[rows] = (ee < 0).any(axis=1)
+
+    for l in range(lmax):
+        nm = min(l + 1, mmax)
+        r = min(l - 1, mmax)                            # rows m <= l - 2
+        if r > 0:
+            a, b = recurrence_coeffs(l, np.arange(r))
+            nxt = (x[None, :] * a[:, None]) * cur[:r] - b[:, None] * prev[:r]
+            prev[:r] = cur[:r]
+            cur[:r] = nxt
+            rescale(0, r)
+        if 1 <= l <= mmax:                              # m = l - 1: P_{m+1, m}
+            m = l - 1
+            prev[m] = cur[m]
+            cur[m] = np.sqrt(2 * m + 3) * x * prev[m]
+            rescale(m, m + 1)
+        if l < mmax:                                    # m = l: sectoral
+            if l > 0:
+                pmm = pmm * np.sqrt((2 * l + 1) / (2 * l)) * s
+                small = (np.abs(pmm) < _TINY) & (pmm != 0)
+                pmm = np.where(small, pmm * _BIG, pmm)
+                emm = emm - small.astype(np.int64)
+            cur[l] = pmm
+            prev[l] = 0.0
+            e[l] = emm
+            scaled[l] = (emm < 0).any()
+        yield l, np.where(e[:nm] == 0, cur[:nm], 0.0)
+
+
 def legpoly_scaled(mmax, lmax, x):
     """Same functions, recurrence in l per m with an exponent carried beside.
 

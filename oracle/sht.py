@@ -30,7 +30,7 @@ variants restate them.
 import numpy as np
 
 from .geometry import RingGrid
-from .legendre import legpoly
+from .legendre import legpoly, legpoly_scaled_lrows
 
 
 def ring_rfft(f, grid, mmax):
@@ -94,24 +94,50 @@ def cm_factors(grid, mmax):
 class SHTOracle:
     """FP64 forward / inverse SHT and adjoints on a ring grid."""
 
-    def __init__(self, grid, lmax, mmax, pct=None):
+    def __init__(self, grid, lmax, mmax, pct=None, streamed=False):
         self.grid = grid
         self.lmax = lmax
         self.mmax = mmax
-        if pct is None:
+        self.streamed = streamed
+        if pct is None and not streamed:
             pct = legpoly(mmax, lmax, np.cos(grid.theta))
-        self.pct = pct  # [mmax, lmax, nring]
+        self.pct = pct  # [mmax, lmax, nring]; None when streamed
 
     @classmethod
-    def healpix(cls, nside, lmax=None, mmax=None, quad_weights="ring"):
+    def healpix(cls, nside, lmax=None, mmax=None, quad_weights="ring", streamed=False):
+        """``streamed``: generate P one m at a time (scaled recurrence) instead
+        of holding the [mmax, lmax, nring] table, for large grids."""
         lmax = lmax or 4 * nside - 1
         mmax = mmax or 2 * nside + 1
-        return cls(RingGrid.healpix(nside, quad_weights), lmax, mmax)
+        return cls(RingGrid.healpix(nside, quad_weights), lmax, mmax, streamed=streamed)
+
+    _LBLOCK = 32
+
+    def _lblocks(self):
+        """(l0, Q) with Q[m, l - l0, k] = P_lm(x_k) for a block of l; zero for m > l."""
+        nx = self.grid.nring
+        buf = None
+        l0 = 0
+        for l, q in legpoly_scaled_lrows(self.mmax, self.lmax, np.cos(self.grid.theta)):
+            if buf is None:
+                buf = np.zeros((self.mmax, self._LBLOCK, nx))
+                l0 = l
+            buf[:q.shape[0], l - l0, :] = q
+            if l - l0 + 1 == self._LBLOCK or l == self.lmax - 1:
+                yield l0, buf[:, :l - l0 + 1, :]
+                buf = None
 
     def legendre_analysis(self, ftm):
         """a[..., l, m] = sum_k P[m, l, k] ftm[..., k, m] (einsum '...km,mlk->...lm')."""
         ftm = np.asarray(ftm, dtype=np.complex128)
         out = np.zeros(ftm.shape[:-2] + (self.lmax, self.mmax), dtype=np.complex128)
+        if self.streamed:  # blocks of l, one batched product over m per block
+            lead = ftm.shape[:-2]
+            fm = np.moveaxis(ftm.reshape((-1,) + ftm.shape[-2:]), -1, 0)  # [m, F, k]
+            for l0, q in self._lblocks():
+                blk = np.matmul(fm, np.swapaxes(q, 1, 2))  # [m, F, lb]
+                out.reshape((-1, self.lmax, self.mmax))[:, l0:l0 + q.shape[1], :] = np.moveaxis(blk, 0, -1)
+            return out.reshape(lead + (self.lmax, self.mmax))
         for m in range(self.mmax):
             out[..., :, m] = ftm[..., :, m] @ self.pct[m].T
         return out
@@ -120,6 +146,13 @@ class SHTOracle:
         """g[..., k, m] = sum_l P[m, l, k] a[..., l, m] (einsum '...lm,mlk->...km')."""
         a = np.asarray(a, dtype=np.complex128)
         out = np.zeros(a.shape[:-2] + (self.grid.nring, self.mmax), dtype=np.complex128)
+        if self.streamed:
+            lead = a.shape[:-2]
+            am = np.moveaxis(a.reshape((-1,) + a.shape[-2:]), -1, 0)  # [m, F, l]
+            acc = np.zeros((self.mmax, am.shape[1], self.grid.nring), dtype=np.complex128)
+            for l0, q in self._lblocks():
+                acc += np.matmul(am[:, :, l0:l0 + q.shape[1]], q.astype(np.complex128))
+            return np.moveaxis(acc, 0, -1).reshape(lead + (self.grid.nring, self.mmax))
         for m in range(self.mmax):
             out[..., :, m] = a[..., :, m] @ self.pct[m]
         return out

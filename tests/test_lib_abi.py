@@ -101,3 +101,23 @@ def test_sht_modules_need_a_gpu():
         cuhpx_amd.SHTCUDA(4)
     with pytest.raises(RuntimeError, match="GPU"):
         cuhpx_amd.iSHTCUDA(4)
+
+
+def test_mixed_and_autotune_reject_bad_arguments():
+    lib = _lib.lib()
+    h = ctypes.c_void_p()
+    plans = (ctypes.c_void_p * 1)(None)
+    counts = (ctypes.c_int64 * 1)(4)
+    assert lib.hpx_sht_mixed_workspace_bytes(1, plans, counts) == -1
+    assert lib.hpx_sht_mixed_workspace_bytes(0, None, None) == 0
+    assert lib.hpx_sht_mixed_create(0, plans, counts, 1, 3, None, None, None, None, 0,
+                                    ctypes.byref(h)) == _lib.HPX_ERR_ARG
+    assert lib.hpx_sht_mixed_create(_lib.MIXED_MAX_PLANS + 1, plans, counts, 1, 3, None, None, None, None, 0,
+                                    ctypes.byref(h)) == _lib.HPX_ERR_ARG
+    assert lib.hpx_sht_mixed_create(1, plans, counts, 1, 0, None, None, None, None, 0,
+                                    ctypes.byref(h)) == _lib.HPX_ERR_ARG
+    assert lib.hpx_sht_mixed_launch(None, None) == _lib.HPX_ERR_ARG
+    lib.hpx_sht_mixed_destroy(None)
+    assert lib.hpx_sht_autotune(None, 1, 4, None, 0, None) == _lib.HPX_ERR_ARG
+    shapes = (ctypes.c_int32 * 8)()
+    assert lib.hpx_sht_tuned_shapes(None, 1, 4, shapes) == _lib.HPX_ERR_ARG

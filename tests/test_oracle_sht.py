@@ -132,6 +132,20 @@ def test_scaled_recurrence_keeps_tiny_values():
 
 
 # ---------------------------------------------------------------- ring DFT and SHT
+
+def test_streamed_oracle_equals_table_oracle():
+    """The per-m streamed tables (for grids whose full table does not fit in
+    memory) give the same transforms as the full-table oracle."""
+    o = S.SHTOracle.healpix(8, 20, 17)
+    o_s = S.SHTOracle.healpix(8, 20, 17, streamed=True)
+    assert o_s.pct is None
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal((2, o.grid.npix))
+    a = o.forward(f)
+    np.testing.assert_allclose(o_s.forward(f), a, rtol=0, atol=1e-13 * np.abs(a).max())
+    g = o.inverse(a)
+    np.testing.assert_allclose(o_s.inverse(a), g, rtol=0, atol=1e-13 * np.abs(g).max())
+
 @pytest.mark.parametrize("nside,mmax", [(2, 5), (4, 9), (4, 20), (8, 17)])
 def test_ring_dft_matches_bruteforce(nside, mmax):
     g = G.RingGrid.healpix(nside)
@@ -272,3 +286,16 @@ def test_cpu_torch_path_restatement_matches_oracle():
     np.testing.assert_allclose(a, o.forward(f), rtol=0, atol=1e-7 * np.abs(a).max())
     back = c.inverse(torch.from_numpy(a)).numpy()
     np.testing.assert_allclose(back, o.inverse(a), rtol=0, atol=1e-7 * np.abs(back).max())
+
+
+def test_l_outer_scaled_recurrence_equals_per_m_recurrence():
+    """legpoly_scaled_lrows (all m advanced together, l outer) reproduces
+    legpoly_scaled bit for bit, deep into the rescaled (underflow) region."""
+    x = np.cos(np.linspace(1e-4, np.pi - 1e-4, 301))
+    for mmax, lmax in [(9, 9), (5, 12), (12, 5), (700, 900)]:
+        ref = L.legpoly_scaled(mmax, lmax, x)
+        got = np.zeros_like(ref)
+        for l, q in L.legpoly_scaled_lrows(mmax, lmax, x):
+            assert q.shape[0] == min(l + 1, mmax)
+            got[:q.shape[0], l, :] = q
+        assert np.array_equal(got, ref), (mmax, lmax)
