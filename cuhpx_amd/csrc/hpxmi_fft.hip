@@ -43,6 +43,14 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 // multiply by -i
 __device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
 
+// LDS position of point x of a wave's sequence: one float2 of padding after
+// every 16 points, so that the strided Stockham writes (stride R points in the
+// first pass) spread over the banks.  Sequences of consecutive waves are
+// wave_stride(M) float2 apart (two more), so that the same point of different
+// waves falls in different banks in the cross-wave store loops.
+__device__ __forceinline__ int pad(int x) { return x + (x >> 4); }
+__host__ __device__ constexpr int wave_stride(int m) { return m + (m >> 4) + 2; }
+
 template <int R>
 __device__ __forceinline__ void dft(float2* v);
 
@@ -130,12 +138,21 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
     const int b = lane + 64 * i;
     if (kNS > 1 && (kNB >= 64 || b < kNB)) w[i] = tw[(b & (kNS - 1)) << tw_shift];
   }
+  // padded positions: for power-of-two strides s, pad(base + r s) =
+  // pad(base) + r s + ((r s) >> 4) when s is a multiple of 16 (reads) or base
+  // is a Stockham output base (writes), so the offsets stay immediate
 #pragma unroll
   for (int i = 0; i < kB; ++i) {
     const int b = lane + 64 * i;
     if (kNB >= 64 || b < kNB) {
+      const int pb = pad(b);
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[b + r * kNB];
+      for (int r = 0; r < R; ++r) {
+        if constexpr (kNB % 16 == 0)
+          v[i][r] = buf[pb + r * kNB + ((r * kNB) >> 4)];
+        else
+          v[i][r] = buf[pad(b + r * kNB)];
+      }
     }
   }
   wave_sync();
@@ -146,9 +163,9 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
       const int jn = b & (kNS - 1);
       if (kNS > 1) twiddle<R>(v[i], w[i]);
       dft<R>(v[i]);
-      float2* dst = buf + (b - jn) * R + jn;
+      const int pbase = pad((b - jn) * R + jn);
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst[r * kNS] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[pbase + r * kNS + ((r * kNS) >> 4)] = v[i][r];
     }
   }
   wave_sync();
@@ -202,16 +219,18 @@ __device__ __forceinline__ void wave_ring_dft(float2* buf, const RingPair& pr,
 #pragma unroll
     for (int i = 0; i < kIt; ++i) {
       const int x = lane + 64 * i;
-      if (x < pr.M) buf[x] = cconj(cmul(buf[x], bh[i]));
+      const int px = pad(lane) + 68 * i;
+      if (x < pr.M) buf[px] = cconj(cmul(buf[px], bh[i]));
     }
     wave_sync();
     wave_fft<LO, HI>(buf, pr.log2M, tw, log2_tab, lane);
   }
 }
 
+template <bool PADDED>
 __device__ __forceinline__ float2 dft_value(const float2* seq, const RingPair& pr, int x,
                                             const float2* __restrict__ chirp) {
-  const float2 v = seq[x];
+  const float2 v = seq[PADDED ? pad(x) : x];
   if (pr.M == pr.nphi) return v;
   return cmul(chirp[x], cconj(v));
 }
@@ -231,6 +250,7 @@ __device__ __forceinline__ void unpack_pair(float2 zm, float2 zc, float2 ph, flo
 // applied to conj(Z) with Z = Xn + i Xs Hermitian-extended; writes positions
 // m and (unless m is DC or Nyquist) N - m of the sequence (chirp-multiplied
 // for Bluestein).
+template <bool PADDED>
 __device__ __forceinline__ void pack_pair(float2* seq, const RingPair& pr, int m, float2 Gn,
                                           float2 Gs, float2 ph_conj, float c, int cmode,
                                           const float2* __restrict__ chirp) {
@@ -246,11 +266,11 @@ __device__ __forceinline__ void pack_pair(float2* seq, const RingPair& pr, int m
   const bool blue = pr.M != N;
   float2 zp = cconj(make_float2(xn.x - xs.y, xn.y + xs.x));
   if (blue) zp = cmul(zp, chirp[m]);
-  seq[m] = zp;
+  seq[PADDED ? pad(m) : m] = zp;
   if (!edge) {
     float2 zn = cconj(make_float2(xn.x + xs.y, xs.x - xn.y));
     if (blue) zn = cmul(zn, chirp[N - m]);
-    seq[N - m] = zn;
+    seq[PADDED ? pad(N - m) : N - m] = zn;
   }
 }
 
@@ -289,7 +309,8 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
   const float2* chirp = a.chirp + pr.chirp_off;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t b = b0 + wave;
-  float2* buf = lds + wave * M;
+  const int stride = wave_stride(M);
+  float2* buf = lds + wave * stride;
   if (b < a.nbatch) {
     const float* fn = f + b * a.npix + pr.off_n;
     const float* fs = f + b * a.npix + (pr.single ? pr.off_n : pr.off_s);
@@ -315,7 +336,7 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
           float2 v = z[u];
           if (pr.single) v.y = 0.f;
           if (blue && j < N) v = cmul(v, c[u]);
-          buf[j] = v;
+          buf[pad(lane) + 68 * (i0 + u)] = v;
         }
       }
     }
@@ -357,8 +378,8 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
         if (b0 + g >= a.nbatch) continue;
         float2 S = make_float2(0.f, 0.f), D = S;
         if (m < K) {
-          const float2* seq = lds + g * M;
-          float2 zm = seq[m], zc = seq[m == 0 ? 0 : N - m];
+          const float2* seq = lds + g * stride;
+          float2 zm = seq[pad(m)], zc = seq[pad(m == 0 ? 0 : N - m)];
           if (blue) {
             zm = cmul(cm[u], cconj(zm));
             zc = cmul(cc[u], cconj(zc));
@@ -386,7 +407,7 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
       float2 Fn = make_float2(0.f, 0.f), Fs = Fn;
       if (m < K) {
         const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
-        unpack_pair(dft_value(buf, pr, m, chirp), dft_value(buf, pr, m == 0 ? 0 : N - m, chirp),
+        unpack_pair(dft_value<true>(buf, pr, m, chirp), dft_value<true>(buf, pr, m == 0 ? 0 : N - m, chirp),
                     ph[m], c, Fn, Fs);
       }
       out[(int64_t)k * a.mmax + m] = Fn;
@@ -414,7 +435,8 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
   const int K = min(a.mmax, N / 2 + 1);
   const float2* ph = a.phase + (int64_t)k * a.mmax;
 
-  for (int idx = threadIdx.x; idx < WAVES * M; idx += kThreadsW) lds[idx] = make_float2(0.f, 0.f);
+  const int stride = wave_stride(M);
+  for (int idx = threadIdx.x; idx < WAVES * stride; idx += kThreadsW) lds[idx] = make_float2(0.f, 0.f);
   __syncthreads();
   if (a.layout == 0) {
     const int64_t pstride = (int64_t)a.kpad * a.ncol;
@@ -440,8 +462,8 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
         const int m = e / WAVES;
         const int g = e - m * WAVES;
         if (e < total && b0 + g < a.nbatch)
-          pack_pair(lds + g * M, pr, m, cadd(S[u], D[u]), csub(S[u], D[u]), cconj(phv[u]), wscale,
-                    a.cmode, chirp);
+          pack_pair<true>(lds + g * stride, pr, m, cadd(S[u], D[u]), csub(S[u], D[u]), cconj(phv[u]),
+                          wscale, a.cmode, chirp);
       }
     }
   } else {
@@ -452,12 +474,12 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
       const float2* in = reinterpret_cast<const float2*>(X) + (b0 + g) * a.nring * (int64_t)a.mmax;
       const float2 Gn = in[(int64_t)k * a.mmax + m];
       const float2 Gs = pr.single ? Gn : in[(int64_t)(a.nring - 1 - k) * a.mmax + m];
-      pack_pair(lds + g * M, pr, m, Gn, Gs, cconj(ph[m]), wscale, a.cmode, chirp);
+      pack_pair<true>(lds + g * stride, pr, m, Gn, Gs, cconj(ph[m]), wscale, a.cmode, chirp);
     }
   }
   __syncthreads();
   if (b >= a.nbatch) return;
-  float2* buf = lds + wave * M;
+  float2* buf = lds + wave * stride;
   wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab, lane);
   float* fr = f + b * a.npix;
   constexpr int kIt = ((1 << HI) + 63) / 64;
@@ -477,7 +499,7 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
     for (int u = 0; u < kBatch; ++u) {
       const int j = lane + 64 * (i0 + u);
       if (j < N) {
-        float2 r = buf[j];
+        float2 r = buf[pad(lane) + 68 * (i0 + u)];
         if (blue) r = cmul(c[u], cconj(r));
         fr[pr.off_n + j] = r.x;
         if (!pr.single) fr[pr.off_s + j] = -r.y;
@@ -588,8 +610,8 @@ __global__ __launch_bounds__(kBlockThreads) void ring_analysis_block_kernel(
     float2 Fn = make_float2(0.f, 0.f), Fs = Fn;
     if (m < K) {
       const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
-      unpack_pair(dft_value(buf, pr, m, chirp), dft_value(buf, pr, m == 0 ? 0 : N - m, chirp), ph[m],
-                  c, Fn, Fs);
+      unpack_pair(dft_value<false>(buf, pr, m, chirp), dft_value<false>(buf, pr, m == 0 ? 0 : N - m, chirp),
+                  ph[m], c, Fn, Fs);
     }
     if (a.layout == 0) {
       const float2 S = pr.single ? Fn : cadd(Fn, Fs);
@@ -633,13 +655,13 @@ __global__ __launch_bounds__(kBlockThreads) void ring_synthesis_block_kernel(
       Gn = in[(int64_t)k * a.mmax + m];
       Gs = pr.single ? Gn : in[(int64_t)(a.nring - 1 - k) * a.mmax + m];
     }
-    pack_pair(buf, pr, m, Gn, Gs, cconj(ph[m]), wscale, a.cmode, chirp);
+    pack_pair<false>(buf, pr, m, Gn, Gs, cconj(ph[m]), wscale, a.cmode, chirp);
   }
   __syncthreads();
   block_ring_dft(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab);
   float* fr = f + b * a.npix;
   for (int j = threadIdx.x; j < N; j += kBlockThreads) {
-    const float2 r = dft_value(buf, pr, j, chirp);
+    const float2 r = dft_value<false>(buf, pr, j, chirp);
     fr[pr.off_n + j] = r.x;
     if (!pr.single) fr[pr.off_s + j] = -r.y;
   }
@@ -672,10 +694,23 @@ template <int WAVES, int C>
 void launch_wave(bool analysis, unsigned grid, size_t lds_bytes, const float* in, float* out,
                  const StageArgs& a, hipStream_t s) {
   constexpr int kLog2 = 10 + C;
+  constexpr int kLo = C == 0 ? 1 : kLog2;
+  auto ka = ring_analysis_wave_kernel<WAVES, kLo, kLog2>;
+  auto ks = ring_synthesis_wave_kernel<WAVES, kLo, kLog2>;
+  // the padded sequences take a little more than 64 KB per work-group
+  // (gfx950 allows 160 KB)
+  static const bool attr_set = [&] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(ka), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(ks), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              160 * 1024);
+    return true;
+  }();
+  (void)attr_set;
   if (analysis)
-    ring_analysis_wave_kernel<WAVES, (C == 0 ? 1 : kLog2), kLog2><<<grid, WAVES * 64, lds_bytes, s>>>(in, out, a);
+    ka<<<grid, WAVES * 64, lds_bytes, s>>>(in, out, a);
   else
-    ring_synthesis_wave_kernel<WAVES, (C == 0 ? 1 : kLog2), kLog2><<<grid, WAVES * 64, lds_bytes, s>>>(in, out, a);
+    ks<<<grid, WAVES * 64, lds_bytes, s>>>(in, out, a);
 }
 
 }  // namespace
@@ -703,7 +738,7 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, const fl
   const int64_t nfg = (nbatch + waves - 1) / waves;
   const int64_t items = (int64_t)n * nfg;
   if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  const size_t lds_bytes = ((size_t)waves << (10 + c)) * sizeof(float2);
+  const size_t lds_bytes = (size_t)waves * wave_stride(1 << (10 + c)) * sizeof(float2);
   const unsigned grid = (unsigned)items;
   switch (c * 16 + waves) {
     case 0 * 16 + 4: launch_wave<4, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
