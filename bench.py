@@ -71,9 +71,57 @@ def algorithmic_figures():
     n_lm = sum(LMAX - m for m in range(MMAX))
     return {
         "legendre_flop_per_map_per_dir": 4.0 * n_lm * n_half,
+        "legendre_io_bytes_per_map_per_dir": 8.0 * nring * MMAX + 8.0 * n_lm,
+        "legendre_table_bytes_per_dir": 4.0 * n_lm * n_half,
         "fft_bytes_per_map_per_dir": 4.0 * npix + 8.0 * nring * MMAX,
         "n_lm": n_lm,
     }
+
+
+# effective dense MFMA peak of each Legendre arithmetic (TFLOP/s of the
+# FP32-equivalent product): the 3-term split issues three BF16 products
+MFMA_PEAK = {"fp32": PEAK["fp32_TFLOPs"], "bf16x3": PEAK["bf16_TFLOPs"] / 3, "bf16": PEAK["bf16_TFLOPs"]}
+
+
+def stage_roofline(name, ms, precision):
+    """Roofline of one stage of the batch: algorithmic bytes (maps, spectra,
+    coefficients, Legendre table) and FP32-equivalent flops; the stage is
+    bound by HBM when its arithmetic intensity is below the ridge of the
+    precision's MFMA peak, else by MFMA."""
+    fig = algorithmic_figures()
+    t = ms * 1e-3
+    if name.startswith("ring"):
+        nbytes = fig["fft_bytes_per_map_per_dir"] * BATCH
+        gbs = nbytes / t / 1e9
+        return {"kernel": name, "bound": "hbm", "achieved": gbs, "peak": PEAK["hbm_GBs"], "unit": "GB/s",
+                "frac": gbs / PEAK["hbm_GBs"], "algorithmic_bytes": nbytes}
+    flop = fig["legendre_flop_per_map_per_dir"] * BATCH
+    nbytes = fig["legendre_io_bytes_per_map_per_dir"] * BATCH + fig["legendre_table_bytes_per_dir"]
+    peak = MFMA_PEAK[precision]
+    ridge = peak * 1e12 / (PEAK["hbm_GBs"] * 1e9)
+    ai = flop / nbytes
+    if ai < ridge:
+        gbs = nbytes / t / 1e9
+        return {"kernel": name, "bound": "hbm", "achieved": gbs, "peak": PEAK["hbm_GBs"], "unit": "GB/s",
+                "frac": gbs / PEAK["hbm_GBs"], "algorithmic_bytes": nbytes, "flop": flop,
+                "intensity_flop_per_byte": ai, "ridge_flop_per_byte": ridge}
+    tf = flop / t / 1e12
+    return {"kernel": name, "bound": "mfma", "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak,
+            "algorithmic_bytes": nbytes, "flop": flop, "intensity_flop_per_byte": ai, "ridge_flop_per_byte": ridge}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_nside256_b64.json")
+
+
+def measured_traffic(stage):
+    """HBM bytes per launch of a stage's kernels from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            data = json.load(fh)
+        return data["stages"][stage]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 class Runner:
@@ -335,20 +383,11 @@ def main():
 
     ms, stage_ms, main_run = runs[args.precision]
     value = BATCH / (ms / 1e3)
-    fig = algorithmic_figures()
     names = ["ring_analysis", "legendre_analysis", "legendre_synthesis", "ring_synthesis"]
-    dom = int(np.argmax(stage_ms))
-    if dom in (1, 2):
-        flop = fig["legendre_flop_per_map_per_dir"] * BATCH
-        peak = PEAK["fp32_TFLOPs"] if args.precision == "fp32" else PEAK["bf16_TFLOPs"]
-        achieved = flop / (stage_ms[dom] * 1e-3) / 1e12
-        roof = {"kernel": names[dom], "bound": "mfma", "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
-    else:
-        nbytes = fig["fft_bytes_per_map_per_dir"] * BATCH
-        achieved = nbytes / (stage_ms[dom] * 1e-3) / 1e9
-        roof = {"kernel": names[dom], "bound": "hbm", "achieved": achieved, "peak": PEAK["hbm_GBs"],
-                "unit": "GB/s", "frac": achieved / PEAK["hbm_GBs"], "traffic": None}
+    stage_roofs = {n: stage_roofline(n, t, args.precision) for n, t in zip(names, stage_ms)}
+    dom = names[int(np.argmax(stage_ms))]
+    roof = dict(stage_roofs[dom])
+    roof["traffic"] = measured_traffic(dom)
 
     result = {
         "metric": "fwd+inv SHT maps/s, nside=256 batch 64: split-BF16 MFMA vs FP32 @ rel-L2<=2e-5",
@@ -370,6 +409,7 @@ def main():
         },
         "stages_ms": dict(zip(names, stage_ms)),
         "roofline": roof,
+        "stage_rooflines": stage_roofs,
         "plan_build_s": plan_s,
     }
     if "fp32" in runs and args.precision != "fp32":
