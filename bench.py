@@ -435,7 +435,9 @@ def main():
     torch.cuda.synchronize()
 
     runs = {}
+    tuned = {}
     for prec in [args.precision, "fp32"] if args.precision != "fp32" else ["fp32"]:
+        tuned[prec] = plan.autotune(prec, BATCH)  # launch shapes measured on this device
         r = Runner(plan, f, prec)
         ms, stage_ms = r.time(args.steps, args.warmup)
         runs[prec] = (ms, stage_ms, r)
@@ -470,6 +472,7 @@ def main():
         "roofline": roof,
         "stage_rooflines": stage_roofs,
         "plan_build_s": plan_s,
+        "tuned_shapes": tuned,
     }
     if "fp32" in runs and args.precision != "fp32":
         ms32 = runs["fp32"][0]

@@ -667,6 +667,15 @@ __global__ __launch_bounds__(kBlockThreads) void ring_synthesis_block_kernel(
   }
 }
 
+// Rows k in [npair, kpad) of the folded spectra (padding of the Legendre
+// GEMM's reduction dimension) are zero, so that the GEMM reads them unguarded.
+__global__ __launch_bounds__(256) void zero_pad_rows_kernel(float* __restrict__ X, int kpad, int npair,
+                                                            int64_t ncol) {
+  float* base = X + ((int64_t)blockIdx.x * kpad + npair) * ncol;
+  const int64_t n = (int64_t)(kpad - npair) * ncol;
+  for (int64_t i = threadIdx.x; i < n; i += 256) base[i] = 0.f;
+}
+
 StageArgs make_args(const ShtPlan& p, int64_t nbatch, int use_weight, float scale, int cmode,
                     int layout) {
   StageArgs a;
@@ -758,6 +767,10 @@ template <bool ANALYSIS>
 int launch_stage(const ShtPlan& p, const float* in, float* out, int64_t nbatch, int use_weight,
                  float scale, int cmode, hipStream_t s, int layout) {
   if (nbatch <= 0) return HPX_OK;
+  if (ANALYSIS && layout == 0 && p.kpad > p.npair) {
+    zero_pad_rows_kernel<<<(unsigned)(2 * p.mmax), 256, 0, s>>>(out, p.kpad, p.npair, col_pad(nbatch));
+    HPX_LAUNCH_CHECK();
+  }
   const FftTune t = p.fft_tune(col_pad(nbatch));
   for (int c = 0; c < kFftClasses; ++c) {
     const int waves = c < 3 ? t.waves[ANALYSIS ? 0 : 1][c] : 1;

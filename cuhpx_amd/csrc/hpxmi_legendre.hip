@@ -31,6 +31,8 @@
 #include "hpxmi_sht_internal.h"
 #include "../../include/hpxmi.h"
 
+#include <algorithm>
+
 namespace hpxmi {
 namespace {
 
@@ -142,19 +144,153 @@ struct FwdLds<HPX_PREC_BF16> {
   __bf16 hi[2][kRowsPerParity][kKTile + kLdsPadH];
 };
 
+// Split a value into BF16 hi and lo (x = hi + lo to ~16 bits more).
+template <int PREC>
+__device__ __forceinline__ void split_to(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    if constexpr (PREC == HPX_PREC_BF16X3) {
+      __bf16 h, l;
+      split(v[q], h, l);
+      hi[q] = h;
+      lo[q] = l;
+    } else {
+      hi[q] = (__bf16)v[q];
+    }
+  }
+}
+
+// One k (or i) chunk of MFMA products for 2 parities x 4 row tiles x CT
+// column tiles: A fragments from LDS (row-major tile, row = lane & 15),
+// B fragments from registers.  Straight-line: row tiles beyond the valid
+// rows multiply zeros.
+template <int PREC, int CT, class LdsT>
+__device__ __forceinline__ void mfma_chunk(const LdsT& L, const float (&bv)[2][CT][8], floatx4 (&acc)[2][4][CT],
+                                           int lane) {
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    if constexpr (PREC == HPX_PREC_FP32) {
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        const int ar = rt * 16 + (lane & 15);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const float av = L.a[p][ar][4 * q + (lane >> 4)];
+#pragma unroll
+          for (int ct = 0; ct < CT; ++ct)
+            acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
+        }
+      }
+    } else {
+      bf16x8 bhi[CT], blo[CT];
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct) split_to<PREC>(bv[p][ct], bhi[ct], blo[ct]);
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) {
+        const int ar = rt * 16 + (lane & 15);
+        const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&L.hi[p][ar][8 * (lane >> 4)]);
+        bf16x8 alo;
+        if constexpr (PREC == HPX_PREC_BF16X3)
+          alo = *reinterpret_cast<const bf16x8*>(&L.lo[p][ar][8 * (lane >> 4)]);
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+          acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi[ct], acc[p][rt][ct], 0, 0, 0);
+          if constexpr (PREC == HPX_PREC_BF16X3) {
+            acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo[ct], acc[p][rt][ct], 0, 0, 0);
+            acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi[ct], acc[p][rt][ct], 0, 0, 0);
+          }
+        }
+      }
+    }
+  }
+}
+
+// k offset (within a chunk) of B value q of a lane: the MFMA B fragment
+// holds k = 4q + lane/16 (FP32, 16x16x4) or k = 8 (lane/16) + q (BF16, 16x16x32).
+template <int PREC>
+__device__ __forceinline__ int frag_k(int q, int lane) {
+  return PREC == HPX_PREC_FP32 ? 4 * q + (lane >> 4) : 8 * (lane >> 4) + q;
+}
+// The same split into a per-lane part and a wave-uniform part.
+template <int PREC>
+__device__ __forceinline__ int frag_k_lane(int lane) {
+  return PREC == HPX_PREC_FP32 ? (lane >> 4) : 8 * (lane >> 4);
+}
+template <int PREC>
+constexpr int frag_k_uniform(int q) {
+  return PREC == HPX_PREC_FP32 ? 4 * q : q;
+}
+
+// Raw buffer access (gfx9 descriptor: 32-bit dwords, bounds-checked against
+// nbytes, out-of-range loads return 0).  B fragments are strided loads whose
+// offsets are wave-uniform except for one lane term: the uniform part goes in
+// the scalar offset, so a chunk's loads need one vector register for address.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const float* base, int64_t nbytes) {
+  const int n = nbytes > 0x7fffffff ? 0x7fffffff : (int)nbytes;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, n, 0x00020000);
+}
+__device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t r, int voff_bytes, int soff_bytes) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff_bytes, soff_bytes, 0));
+}
+
+// Write the two 64-row x (64 CT)-column accumulator tiles of a work-group
+// (one per parity) through LDS, so that each store instruction writes whole
+// row segments, 16 bytes per lane.  T: LDS scratch of at least
+// 64 * (64 CT + 4) floats, 16-byte aligned (the table buffers, free after
+// the main loop); row(p, r): destination of local row r of parity p, or
+// nullptr; only the first ncol_left columns of a row exist.  All threads of
+// the work-group call it.
+template <int CT, class RowFn>
+__device__ __forceinline__ void store_tiles_lds(const floatx4 (&acc)[2][4][CT], float* T, int wave, int lane,
+                                                int64_t ncol_left, RowFn row) {
+  constexpr int kW = 64 * CT;
+  constexpr int kS = kW + 4;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          T[(rt * 16 + 4 * (lane >> 4) + r) * kS + wave * 16 * CT + ct * 16 + (lane & 15)] = acc[p][rt][ct][r];
+    __syncthreads();
+    for (int v = threadIdx.x; v < 64 * (kW / 4); v += kThreads) {
+      const int r = v / (kW / 4);
+      const int c4 = v - r * (kW / 4);
+      float* dst = row(p, r);
+      if (dst != nullptr && 4 * c4 < ncol_left)
+        *reinterpret_cast<float4*>(dst + 4 * c4) = *reinterpret_cast<const float4*>(&T[r * kS + 4 * c4]);
+    }
+  }
+}
+
+// Occupancy target of the GEMM kernels (waves per SIMD) that their register
+// use allows without spilling.
+constexpr int gemm_min_waves(int prec, int ct) { return ct == 1 ? 4 : (prec == HPX_PREC_FP32 ? 3 : 2); }
+
 template <int PREC, int CT>
-__global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
+__global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_fwd_kernel(
     const float* __restrict__ X, float* __restrict__ aT, const float* __restrict__ P,
     const int64_t* __restrict__ p_off, const int* __restrict__ nl,
-    const int2* __restrict__ items, int nitems, int npair, int kpad, int64_t ncol) {
-  __shared__ FwdLds<PREC> lds[2];
-  const int item = xcd_local_index(blockIdx.x, gridDim.x);
-  if (item >= nitems) return;
+    const int2* __restrict__ items, int nitems, int ncb, int kpad, int64_t ncol) {
+  // table tiles (double buffer) during the main loop, the output tile after it
+  constexpr size_t kTileBytes = sizeof(float) * 64 * (64 * CT + 4);
+  constexpr size_t kLdsBytes = 2 * sizeof(FwdLds<PREC>) > kTileBytes ? 2 * sizeof(FwdLds<PREC>) : kTileBytes;
+  __shared__ __align__(16) unsigned char smem[kLdsBytes];
+  FwdLds<PREC>* lds = reinterpret_cast<FwdLds<PREC>*>(smem);
+  // work index = (item, column block), column block fastest: the column
+  // blocks of one table tile run side by side on one XCD and share the tile
+  const int idx = xcd_local_index(blockIdx.x, gridDim.x);
+  if (idx >= nitems * ncb) return;
+  const int item = idx / ncb;
+  const int cb = idx - item * ncb;
   const int m = items[item].x;
   const int jb = items[item].y;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int col0 = blockIdx.y * (64 * CT) + wave * (16 * CT);
+  const int col0 = cb * (64 * CT) + wave * (16 * CT);
   const bool wave_active = col0 < ncol;
   const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
   const int padp0 = (nlp[0] + kRowPad - 1) / kRowPad * kRowPad;
@@ -162,7 +298,7 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
   const int row0 = jb * kRowsPerParity;
   const float* Pm = P + p_off[m];
   const int64_t pstride = (int64_t)kpad * ncol;
-  const float* Xm = X + (int64_t)m * 2 * pstride;
+  const int nc = (int)ncol;
 
   floatx4 acc[2][4][CT];
 #pragma unroll
@@ -179,8 +315,11 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
   const int srow = row0 + sr;
   const bool srow_ok = srow < (sp ? padp1 : padp0);
   const float* Psrc = Pm + (sp ? (int64_t)padp0 * kpad : 0) + (int64_t)srow * kpad + sh;
+  // B fragments from X[m] (both parities): the rows beyond the last ring
+  // pair are zero, so no k bound check
+  const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + (int64_t)m * 2 * pstride, 2 * pstride * 4);
+  const int bvoff = (frag_k_lane<PREC>(lane) * nc + col0 + (lane & 15)) * 4;
 
-  // table tile (16 floats of one row) -> registers
   auto load_a = [&](int kc, float* v) {
     if (srow_ok) {
 #pragma unroll
@@ -196,7 +335,6 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
       for (int q = 0; q < 16; ++q) v[q] = 0.f;
     }
   };
-  // registers -> LDS (split into BF16 hi / lo here for the BF16 paths)
   auto store_a = [&](FwdLds<PREC>& L, const float* v) {
     if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
@@ -209,167 +347,87 @@ __global__ __launch_bounds__(kThreads) void legendre_fwd_kernel(
       for (int q = 0; q < 16; ++q) L.hi[sp][sr][sh + q] = (__bf16)v[q];
     }
   };
-  // this wave's B operand (spectra) for one k chunk, in MFMA lane order
   auto load_b = [&](int kc, float (&bv)[2][CT][8]) {
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int col = col0 + ct * 16 + (lane & 15);
+      for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          int kk;
-          if constexpr (PREC == HPX_PREC_FP32)
-            kk = kc + 4 * q + (lane >> 4);
-          else
-            kk = kc + 8 * (lane >> 4) + q;
-          bv[p][ct][q] = kk < npair ? Xm[p * pstride + (int64_t)kk * ncol + col] : 0.f;
-        }
-      }
-  };
-  auto compute = [&](const FwdLds<PREC>& L, const float (&bv)[2][CT][8]) {
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      if constexpr (PREC == HPX_PREC_FP32) {
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
-          if (row0 + rt * 16 >= nlp[p]) continue;
-          const int ar = rt * 16 + (lane & 15);
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const float av = L.a[p][ar][4 * q + (lane >> 4)];
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
-              acc[p][rt][ct] =
-                  __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
-          }
-        }
-      } else {
-        bf16x8 bhi[CT], blo[CT];
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            __bf16 h, l;
-            split(bv[p][ct][q], h, l);
-            bhi[ct][q] = h;
-            blo[ct][q] = l;
-          }
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
-          if (row0 + rt * 16 >= nlp[p]) continue;
-          const int ar = rt * 16 + (lane & 15);
-          const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&L.hi[p][ar][8 * (lane >> 4)]);
-          bf16x8 alo;
-          if constexpr (PREC == HPX_PREC_BF16X3)
-            alo = *reinterpret_cast<const bf16x8*>(&L.lo[p][ar][8 * (lane >> 4)]);
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct) {
-            acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi[ct], acc[p][rt][ct], 0, 0, 0);
-            if constexpr (PREC == HPX_PREC_BF16X3) {
-              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo[ct], acc[p][rt][ct], 0, 0, 0);
-              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi[ct], acc[p][rt][ct], 0, 0, 0);
-            }
-          }
-        }
-      }
-    }
+        for (int q = 0; q < 8; ++q)
+          bv[p][ct][q] = buffer_load(xr, bvoff,
+                                     (int)((p * pstride + (int64_t)(kc + frag_k_uniform<PREC>(q)) * nc + ct * 16) * 4));
   };
 
-  // software pipeline: chunk c+1's loads are in flight while chunk c is
-  // multiplied; the table tile is double-buffered in LDS (one barrier per chunk)
+  // software pipeline over k chunks with two register sets for B and two
+  // LDS buffers for the table tile (one barrier per chunk, no copies)
   const int nchunk = kpad / kKTile;
   float pa[16];
-  float bv[2][CT][8];
+  float b0[2][CT][8], b1[2][CT][8];
   load_a(0, pa);
-  if (wave_active) load_b(0, bv);
+  if (wave_active) load_b(0, b0);
   store_a(lds[0], pa);
   __syncthreads();
-  for (int c = 0; c < nchunk; ++c) {
-    const bool more = c + 1 < nchunk;
-    float pn[16];
-    float bn[2][CT][8];
-    if (more) {
-      load_a((c + 1) * kKTile, pn);
-      if (wave_active) load_b((c + 1) * kKTile, bn);
+  for (int c = 0; c < nchunk; c += 2) {
+    if (c + 1 < nchunk) {
+      load_a((c + 1) * kKTile, pa);
+      if (wave_active) load_b((c + 1) * kKTile, b1);
     }
-    if (wave_active) compute(lds[c & 1], bv);
-    if (more) {
-      store_a(lds[(c + 1) & 1], pn);
-#pragma unroll
-      for (int p = 0; p < 2; ++p)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-          for (int q = 0; q < 8; ++q) bv[p][ct][q] = bn[p][ct][q];
+    if (wave_active) mfma_chunk<PREC, CT>(lds[0], b0, acc, lane);
+    if (c + 1 < nchunk) store_a(lds[1], pa);
+    __syncthreads();
+    if (c + 1 >= nchunk) break;
+    if (c + 2 < nchunk) {
+      load_a((c + 2) * kKTile, pa);
+      if (wave_active) load_b((c + 2) * kKTile, b0);
     }
+    if (wave_active) mfma_chunk<PREC, CT>(lds[1], b1, acc, lane);
+    if (c + 2 < nchunk) store_a(lds[0], pa);
     __syncthreads();
   }
 
-  if (!wave_active) return;
-  // ---- write this m's rows of aT: 16 lanes store 64 contiguous bytes of a row
-  float* Am = aT + (p_off[m] / kpad) * ncol;
-#pragma unroll
-  for (int p = 0; p < 2; ++p)
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int col = col0 + ct * 16 + (lane & 15);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = row0 + rt * 16 + 4 * (lane >> 4) + r;
-          if (i < nlp[p]) Am[(int64_t)((p ? padp0 : 0) + i) * ncol + col] = acc[p][rt][ct][r];
-        }
-      }
+  // ---- this m's rows of aT (rows row0.. of each parity, this work-group's columns)
+  float* Ab = aT + (p_off[m] / kpad) * ncol + cb * (64 * CT);
+  store_tiles_lds<CT>(acc, reinterpret_cast<float*>(smem), wave, lane, ncol - cb * (64 * CT),
+                      [&](int p, int r) -> float* {
+                        const int i = row0 + r;
+                        return i < nlp[p] ? Ab + (int64_t)((p ? padp0 : 0) + i) * ncol : nullptr;
+                      });
 }
 
 // ---------------------------------------------------------------------------
 // Inverse: Y[m][p][k][col] = sum_i P[m][p][i][k] aT[m][p][i][col]
 // ---------------------------------------------------------------------------
 
-template <int PREC>
-struct InvLds;
-
-template <>
-struct InvLds<HPX_PREC_FP32> {
-  float a[2][kKBlock][kKTile + kLdsPadF];
-};
-
-template <>
-struct InvLds<HPX_PREC_BF16X3> {
-  __bf16 hi[2][kKBlock][kKTile + kLdsPadH];
-  __bf16 lo[2][kKBlock][kKTile + kLdsPadH];
-};
-
-template <>
-struct InvLds<HPX_PREC_BF16> {
-  __bf16 hi[2][kKBlock][kKTile + kLdsPadH];
-};
-
 template <int PREC, int CT>
-__global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
-    const float* __restrict__ aT, float* __restrict__ X, const float* __restrict__ P,
-    const int64_t* __restrict__ p_off, const int* __restrict__ nl, int nkb, int nitems, int kpad,
-    int64_t ncol) {
-  __shared__ InvLds<PREC> lds[2];
+__global__ __launch_bounds__(kThreads, CT == 1 ? 4 : 2) void legendre_inv_kernel(
+    const float* __restrict__ aT, float* __restrict__ X, const float* __restrict__ PT,
+    const int64_t* __restrict__ p_off, const int* __restrict__ nl, int nkb, int nitems, int ncb, int kpad,
+    int64_t ncol, int64_t at_rows) {
+  // same tiling as the forward kernel with the roles of i and k exchanged:
+  // rows = 64 ring pairs, reduction over the rows i of aT, A tiles from the
+  // transposed table PT[m][p][k][i]
+  // table tiles (double buffer) during the main loop, the output tile after it
+  constexpr size_t kTileBytes = sizeof(float) * 64 * (64 * CT + 4);
+  constexpr size_t kLdsBytes = 2 * sizeof(FwdLds<PREC>) > kTileBytes ? 2 * sizeof(FwdLds<PREC>) : kTileBytes;
+  __shared__ __align__(16) unsigned char smem[kLdsBytes];
+  FwdLds<PREC>* lds = reinterpret_cast<FwdLds<PREC>*>(smem);
   // m outer, ring-pair block inner: the nkb work-groups of one m read the same
   // coefficient rows aT[m] and land on one XCD (contiguous item ranges)
-  const int item = xcd_local_index(blockIdx.x, gridDim.x);
-  if (item >= nitems) return;
+  const int idx = xcd_local_index(blockIdx.x, gridDim.x);
+  if (idx >= nitems * ncb) return;
+  const int item = idx / ncb;
+  const int cb = idx - item * ncb;
   const int m = item / nkb;
   const int kb = item - m * nkb;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int col0 = blockIdx.y * (64 * CT) + wave * (16 * CT);
+  const int col0 = cb * (64 * CT) + wave * (16 * CT);
   const bool wave_active = col0 < ncol;
   const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
-  const int padp0 = (nlp[0] + kRowPad - 1) / kRowPad * kRowPad;
-  const int padp1 = (nlp[1] + kRowPad - 1) / kRowPad * kRowPad;
+  const int padp[2] = {(nlp[0] + kRowPad - 1) / kRowPad * kRowPad, (nlp[1] + kRowPad - 1) / kRowPad * kRowPad};
   const int nmax = nlp[0] > nlp[1] ? nlp[0] : nlp[1];
-  const float* Pm = P + p_off[m];
-  const int64_t pstride = (int64_t)kpad * ncol;
   const int k0 = kb * kKBlock;
+  const int nc = (int)ncol;
 
   floatx4 acc[2][4][CT];
 #pragma unroll
@@ -379,18 +437,20 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) acc[p][rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  // staging: 2 parities x 32 rows i x 4 quarters of 16 ring pairs
+  // staging: 2 parities x 64 ring pairs x 2 halves of 16 i
   const int sp = threadIdx.x >> 7;
-  const int si = (threadIdx.x >> 2) & 31;
-  const int sq = (threadIdx.x & 3) * 16;
-  const int spad = sp ? padp1 : padp0;
-  const float* Pbase = Pm + (sp ? (int64_t)padp0 * kpad : 0);
+  const int sr = (threadIdx.x >> 1) & 63;
+  const int sh = (threadIdx.x & 1) * 16;
+  const int spad = padp[sp];
+  const float* Pbase = PT + p_off[m] + (sp ? (int64_t)padp[0] * kpad : 0) + (int64_t)sr * spad + sh;
+  const int64_t row0m = p_off[m] / kpad;
+  const __amdgpu_buffer_rsrc_t ar = buffer_rsrc(aT + row0m * ncol, (at_rows + kAtSlackRows - row0m) * ncol * 4);
+  const int bvoff = (frag_k_lane<PREC>(lane) * nc + col0 + (lane & 15)) * 4;
 
+  // table tile: this block's 64 ring pairs, i chunk starting at ic
   auto load_a = [&](int ic, float* v) {
-    const int irow = ic + si;
-    const int kk = k0 + sq;
-    if (irow < spad && kk < kpad) {
-      const float* src = Pbase + (int64_t)irow * kpad + kk;
+    if (k0 + sr < kpad && ic + sh < spad) {
+      const float* src = Pbase + (int64_t)k0 * spad + ic;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
@@ -404,130 +464,98 @@ __global__ __launch_bounds__(kThreads) void legendre_inv_kernel(
       for (int q = 0; q < 16; ++q) v[q] = 0.f;
     }
   };
-  auto store_a = [&](InvLds<PREC>& L, const float* v) {
+  auto store_a = [&](FwdLds<PREC>& L, const float* v) {
     if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) L.a[sp][sq + q][si] = v[q];
+      for (int q = 0; q < 16; ++q) L.a[sp][sr][sh + q] = v[q];
     } else if constexpr (PREC == HPX_PREC_BF16X3) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) split(v[q], L.hi[sp][sq + q][si], L.lo[sp][sq + q][si]);
+      for (int q = 0; q < 16; ++q) split(v[q], L.hi[sp][sr][sh + q], L.lo[sp][sr][sh + q]);
     } else {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) L.hi[sp][sq + q][si] = (__bf16)v[q];
+      for (int q = 0; q < 16; ++q) L.hi[sp][sr][sh + q] = (__bf16)v[q];
     }
   };
-  const float* Am = aT + (p_off[m] / kpad) * ncol;
+  // B fragments: rows i of aT (this m's rows of parity p); rows at or beyond
+  // the parity's row count hold other data, so they enter as zero
+  // B fragments: rows i of aT (this m's rows of parity p).  Rows beyond the
+  // parity's count are zero padding, the other parity's or the next m's
+  // rows, or the zero slack after aT: finite values that meet zero table
+  // entries (PT is zero for i >= the padded count), so no bound check.
   auto load_b = [&](int ic, float (&bv)[2][CT][8]) {
 #pragma unroll
     for (int p = 0; p < 2; ++p)
 #pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int col = col0 + ct * 16 + (lane & 15);
-        const float* src = Am + (int64_t)(p ? padp0 : 0) * ncol + col;
+      for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          int i;
-          if constexpr (PREC == HPX_PREC_FP32)
-            i = ic + 4 * q + (lane >> 4);
-          else
-            i = ic + 8 * (lane >> 4) + q;
-          bv[p][ct][q] = i < nlp[p] ? src[(int64_t)i * ncol] : 0.f;
-        }
-      }
-  };
-  auto compute = [&](const InvLds<PREC>& L, const float (&bv)[2][CT][8], int ic) {
-#pragma unroll
-    for (int p = 0; p < 2; ++p) {
-      if (ic >= nlp[p]) continue;
-      if constexpr (PREC == HPX_PREC_FP32) {
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
-          const int ar = rt * 16 + (lane & 15);
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            const float av = L.a[p][ar][4 * q + (lane >> 4)];
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
-              acc[p][rt][ct] =
-                  __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[p][ct][q], acc[p][rt][ct], 0, 0, 0);
-          }
-        }
-      } else {
-        bf16x8 bhi[CT], blo[CT];
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            __bf16 h, l;
-            split(bv[p][ct][q], h, l);
-            bhi[ct][q] = h;
-            blo[ct][q] = l;
-          }
-#pragma unroll
-        for (int rt = 0; rt < 4; ++rt) {
-          const int ar = rt * 16 + (lane & 15);
-          const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&L.hi[p][ar][8 * (lane >> 4)]);
-          bf16x8 alo;
-          if constexpr (PREC == HPX_PREC_BF16X3)
-            alo = *reinterpret_cast<const bf16x8*>(&L.lo[p][ar][8 * (lane >> 4)]);
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct) {
-            acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, bhi[ct], acc[p][rt][ct], 0, 0, 0);
-            if constexpr (PREC == HPX_PREC_BF16X3) {
-              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, blo[ct], acc[p][rt][ct], 0, 0, 0);
-              acc[p][rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, bhi[ct], acc[p][rt][ct], 0, 0, 0);
-            }
-          }
-        }
-      }
-    }
+        for (int q = 0; q < 8; ++q)
+          bv[p][ct][q] = buffer_load(
+              ar, bvoff, (int)(((int64_t)((p ? padp[0] : 0) + ic + frag_k_uniform<PREC>(q)) * nc + ct * 16) * 4));
   };
 
-  const int nchunk = (nmax + kKTile - 1) / kKTile;
+  const int nchunk = (nmax + kKTile - 1) / kKTile;  // 0 for m >= lmax: zero spectra
   if (nchunk > 0) {
     float pa[16];
-    float bv[2][CT][8];
+    float b0[2][CT][8], b1[2][CT][8];
     load_a(0, pa);
-    if (wave_active) load_b(0, bv);
+    if (wave_active) load_b(0, b0);
     store_a(lds[0], pa);
     __syncthreads();
-    for (int c = 0; c < nchunk; ++c) {
-      const bool more = c + 1 < nchunk;
-      float pn[16];
-      float bn[2][CT][8];
-      if (more) {
-        load_a((c + 1) * kKTile, pn);
-        if (wave_active) load_b((c + 1) * kKTile, bn);
+    for (int c = 0; c < nchunk; c += 2) {
+      if (c + 1 < nchunk) {
+        load_a((c + 1) * kKTile, pa);
+        if (wave_active) load_b((c + 1) * kKTile, b1);
       }
-      if (wave_active) compute(lds[c & 1], bv, c * kKTile);
-      if (more) {
-        store_a(lds[(c + 1) & 1], pn);
-#pragma unroll
-        for (int p = 0; p < 2; ++p)
-#pragma unroll
-          for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-            for (int q = 0; q < 8; ++q) bv[p][ct][q] = bn[p][ct][q];
+      if (wave_active) mfma_chunk<PREC, CT>(lds[0], b0, acc, lane);
+      if (c + 1 < nchunk) store_a(lds[1], pa);
+      __syncthreads();
+      if (c + 1 >= nchunk) break;
+      if (c + 2 < nchunk) {
+        load_a((c + 2) * kKTile, pa);
+        if (wave_active) load_b((c + 2) * kKTile, b0);
       }
+      if (wave_active) mfma_chunk<PREC, CT>(lds[1], b1, acc, lane);
+      if (c + 2 < nchunk) store_a(lds[0], pa);
       __syncthreads();
     }
   }
 
-  if (!wave_active) return;
-  float* Xm = X + (int64_t)m * 2 * pstride;
-#pragma unroll
-  for (int p = 0; p < 2; ++p)
-#pragma unroll
-    for (int rt = 0; rt < 4; ++rt)
-#pragma unroll
-      for (int ct = 0; ct < CT; ++ct) {
-        const int col = col0 + ct * 16 + (lane & 15);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int k = k0 + rt * 16 + 4 * (lane >> 4) + r;
-          if (k < kpad) Xm[p * pstride + (int64_t)k * ncol + col] = acc[p][rt][ct][r];
-        }
-      }
+  // rows k0..k0+63 of X[m][p], this work-group's columns
+  const int64_t pstride = (int64_t)kpad * ncol;
+  float* Xb = X + (int64_t)m * 2 * pstride + (int64_t)k0 * ncol + cb * (64 * CT);
+  store_tiles_lds<CT>(acc, reinterpret_cast<float*>(smem), wave, lane, ncol - cb * (64 * CT),
+                      [&](int p, int r) -> float* {
+                        return k0 + r < kpad ? Xb + p * pstride + (int64_t)r * ncol : nullptr;
+                      });
+}
+
+// PT[m][p][k][i] = P[m][p][i][k]: the table in the inverse GEMM's operand
+// layout (rows = ring pairs, i contiguous, padded to 16 like the rows of P),
+// so that its tiles stage into LDS without a transposition.
+__global__ __launch_bounds__(kThreads) void table_transpose_kernel(const float* __restrict__ P,
+                                                                   float* __restrict__ PT,
+                                                                   const int64_t* __restrict__ p_off,
+                                                                   const int* __restrict__ nl, int kpad) {
+  __shared__ float t[32][33];
+  const int m = blockIdx.z >> 1;
+  const int par = blockIdx.z & 1;
+  const int pad0 = (nl[2 * m] + kRowPad - 1) / kRowPad * kRowPad;
+  const int padp = (nl[2 * m + par] + kRowPad - 1) / kRowPad * kRowPad;
+  const int i0 = blockIdx.y * 32;
+  const int kk0 = blockIdx.x * 32;
+  if (i0 >= padp) return;
+  const int64_t base = p_off[m] + (par ? (int64_t)pad0 * kpad : 0);
+  const int tx = threadIdx.x & 31;
+  const int ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += kThreads / 32) {
+    const int i = i0 + r;
+    t[r][tx] = i < padp ? P[base + (int64_t)i * kpad + kk0 + tx] : 0.f;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += kThreads / 32) {
+    const int i = i0 + tx;
+    if (i < padp) PT[base + (int64_t)(kk0 + r) * padp + i] = t[tx][r];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -591,6 +619,33 @@ __global__ __launch_bounds__(kThreads) void coeff_to_user_kernel(
   }
 }
 
+// Zero the padding rows of aT (rows [nl, padded count) of each (m, parity))
+// and the slack rows after the last one; block = (m, parity), the last
+// block also clears the slack.
+__global__ __launch_bounds__(kThreads) void zero_coeff_pad_kernel(float* __restrict__ aT,
+                                                                  const int64_t* __restrict__ p_off,
+                                                                  const int* __restrict__ nl, int kpad,
+                                                                  int mmax, int64_t at_rows, int64_t ncol) {
+  const int m = blockIdx.x >> 1;
+  const int par = blockIdx.x & 1;
+  int64_t r0, r1;
+  if (m < mmax) {
+    const int n0 = nl[2 * m];
+    const int pad0 = (n0 + kRowPad - 1) / kRowPad * kRowPad;
+    const int n = nl[2 * m + par];
+    const int pad = (n + kRowPad - 1) / kRowPad * kRowPad;
+    const int64_t base = p_off[m] / kpad + (par ? pad0 : 0);
+    r0 = base + n;
+    r1 = base + pad;
+  } else {
+    r0 = at_rows;
+    r1 = at_rows + kAtSlackRows;
+  }
+  const int64_t n = (r1 - r0) * ncol;
+  float* dst = aT + r0 * ncol;
+  for (int64_t i = threadIdx.x; i < n; i += kThreads) dst[i] = 0.f;
+}
+
 // aT from a[b][l][m] (rows m <= l only); columns of fields >= nbatch are zero.
 __global__ __launch_bounds__(kThreads) void user_to_coeff_kernel(
     const float* __restrict__ alm, float* __restrict__ aT, const int64_t* __restrict__ p_off,
@@ -637,6 +692,18 @@ int launch_legendre_table(ShtPlan& p, const double* d_ct, hipStream_t s) {
   legendre_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
                                                   p.lmax);
   HPX_LAUNCH_CHECK();
+  if (p.d_pt != nullptr && p.mmax > 0) {
+    int max_pad = 0;
+    for (int m = 0; m < p.mmax; ++m)
+      for (int par = 0; par < 2; ++par)
+        max_pad = std::max(max_pad, (p.nl[2 * m + par] + kRowPad - 1) / kRowPad * kRowPad);
+    if (max_pad > 0) {
+      if (2 * p.mmax > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "mmax too large");
+      const dim3 tg((unsigned)(p.kpad / 32), (unsigned)((max_pad + 31) / 32), (unsigned)(2 * p.mmax));
+      table_transpose_kernel<<<tg, kThreads, 0, s>>>(p.d_p, p.d_pt, p.d_p_off, p.d_nl, p.kpad);
+      HPX_LAUNCH_CHECK();
+    }
+  }
   return HPX_OK;
 }
 
@@ -649,13 +716,12 @@ int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, flo
                     int64_t nbatch, hipStream_t s) {
   if (nbatch <= 0 || p.n_fwd_items == 0) return HPX_OK;
   const int64_t ncol = col_pad(nbatch);
-  const int64_t ycols = (ncol + 64 * ct - 1) / (64 * ct);
-  if (ycols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  const dim3 grid(xcd_grid(p.n_fwd_items), (unsigned)ycols);
-#define HPX_FWD(PR, CT)                                                                      \
-  legendre_fwd_kernel<PR, CT><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl,     \
-                                                        p.d_fwd_items, p.n_fwd_items, p.npair, \
-                                                        p.kpad, ncol)
+  const int64_t ncb = (ncol + 64 * ct - 1) / (64 * ct);
+  if ((int64_t)p.n_fwd_items * ncb > 0x7ffffff0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  const dim3 grid(xcd_grid((int64_t)p.n_fwd_items * ncb));
+#define HPX_FWD(PR, CT)                                                                  \
+  legendre_fwd_kernel<PR, CT><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, \
+                                                        p.d_fwd_items, p.n_fwd_items, (int)ncb, p.kpad, ncol)
   switch (precision * 4 + ct) {
     case HPX_PREC_FP32 * 4 + 1: HPX_FWD(HPX_PREC_FP32, 1); break;
     case HPX_PREC_FP32 * 4 + 2: HPX_FWD(HPX_PREC_FP32, 2); break;
@@ -674,14 +740,14 @@ int launch_gemm_inv(const ShtPlan& p, int precision, int ct, const float* aT, fl
                     int64_t nbatch, hipStream_t s) {
   if (nbatch <= 0) return HPX_OK;
   const int64_t ncol = col_pad(nbatch);
-  const int64_t ycols = (ncol + 64 * ct - 1) / (64 * ct);
-  if (ycols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  const int64_t ncb = (ncol + 64 * ct - 1) / (64 * ct);
   const int nkb = (p.kpad + kKBlock - 1) / kKBlock;
   const int nitems = nkb * p.mmax;
-  const dim3 grid(xcd_grid(nitems), (unsigned)ycols);
+  if ((int64_t)nitems * ncb > 0x7ffffff0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  const dim3 grid(xcd_grid((int64_t)nitems * ncb));
 #define HPX_INV(PR, CT)                                                                           \
-  legendre_inv_kernel<PR, CT><<<grid, kThreads, 0, s>>>(aT, X, p.d_p, p.d_p_off, p.d_nl, nkb, nitems, \
-                                                        p.kpad, ncol)
+  legendre_inv_kernel<PR, CT><<<grid, kThreads, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, nkb, nitems, \
+                                                        (int)ncb, p.kpad, ncol, p.at_rows)
   switch (precision * 4 + ct) {
     case HPX_PREC_FP32 * 4 + 1: HPX_INV(HPX_PREC_FP32, 1); break;
     case HPX_PREC_FP32 * 4 + 2: HPX_INV(HPX_PREC_FP32, 2); break;
@@ -734,6 +800,9 @@ int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float
   if (precision < 0 || precision > 2) HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
   float* aT = ws + x_floats(p, nbatch);
   const GemmTune t = p.gemm_tune(precision, col_pad(nbatch));
+  zero_coeff_pad_kernel<<<(unsigned)(2 * p.mmax + 1), kThreads, 0, s>>>(aT, p.d_p_off, p.d_nl, p.kpad, p.mmax,
+                                                                        p.at_rows, col_pad(nbatch));
+  HPX_LAUNCH_CHECK();
   int rc = launch_user_to_coeff(p, alm, aT, nbatch, s);
   if (rc != HPX_OK) return rc;
   return launch_gemm_inv(p, precision, t.inv_ct, aT, ws, nbatch, s);

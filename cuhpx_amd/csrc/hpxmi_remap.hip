@@ -10,7 +10,7 @@
 // MI355X design: instead of materialising a pix array and scattering every
 // row through it (4 cudaMalloc'd index arrays, 4 launches, uncoalesced writes
 // in the reference), one launch walks the map in tiles.  A tile is a
-// TB x TB block of one base face (TB = min(nside, 64)), which is a contiguous
+// TB x TB block of one base face (TB = min(nside, 64 or 128)), which is a contiguous
 // Morton range in NEST order, TB rows of TB in the flat layout, and 2*TB-1
 // contiguous ring segments in RING order.  Each work-group computes the
 // source and destination index of its tile's pixels once, in registers, and
@@ -23,9 +23,7 @@
 
 namespace {
 
-constexpr int kThreads = 256;
-constexpr int kMaxTile = 64;                                  // TB
-constexpr int kPerThread = kMaxTile * kMaxTile / kThreads;    // 16
+constexpr int kPerThread = 16;  // pixels per thread of a tile
 
 enum Scheme { kNest = 0, kRing = 1, kFlat = 2 };
 
@@ -164,12 +162,19 @@ __device__ __forceinline__ void element(const Side& side, const TileGeom& g, int
   }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void remap_tiles(
+// TBMAX: largest tile side (64 or 128; the work-group has TBMAX^2 / 16
+// threads).  Ring segments in a tile average TB/2 pixels, so the RING kinds
+// use 128-pixel tiles (two 64 KB LDS buffers for 4-byte pixels) to write or
+// read longer runs.  NT: non-temporal row loads and stores, which help when
+// both sides stream whole rows of the tile (NEST <-> flat) and hurt the
+// partial-line runs of the RING side, which the L2 has to merge.
+template <typename T, int TBMAX, bool NT>
+__global__ __launch_bounds__(TBMAX * TBMAX / kPerThread) void remap_tiles(
     const T* __restrict__ in, T* __restrict__ out, int64_t nside, int order,
     int tb, int log_tb, int tiles_per_face, int64_t nrows, int rows_per_chunk,
     Side src, Side dst) {
-  __shared__ T lds[2][kMaxTile * kMaxTile];
+  constexpr int kThreads = TBMAX * TBMAX / kPerThread;
+  __shared__ T lds[2][TBMAX * TBMAX];
 
   const int ntiles = gridDim.x;
   const int tile = xcd_local_index(blockIdx.x, ntiles);
@@ -233,14 +238,19 @@ __global__ __launch_bounds__(kThreads) void remap_tiles(
     T v[kPerThread];
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i)
-      if (src_q[i] >= 0) v[i] = src_row[src_i[i]];
+      if (src_q[i] >= 0) v[i] = NT ? __builtin_nontemporal_load(src_row + src_i[i]) : src_row[src_i[i]];
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i)
       if (src_q[i] >= 0) lds[buf][src_q[i]] = v[i];
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i)
-      if (dst_q[i] >= 0) dst_row[dst_i[i]] = lds[buf][dst_q[i]];
+      if (dst_q[i] >= 0) {
+        if constexpr (NT)
+          __builtin_nontemporal_store(lds[buf][dst_q[i]], dst_row + dst_i[i]);
+        else
+          dst_row[dst_i[i]] = lds[buf][dst_q[i]];
+      }
     buf ^= 1;
   }
 }
@@ -268,28 +278,42 @@ Side flat_side(int origin, int cw) {
   return s;
 }
 
-template <typename T>
-int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Side dst,
-           hipStream_t stream) {
+template <typename T, int TBMAX, bool NT>
+int launch_tiles(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Side dst,
+                 hipStream_t stream) {
+  constexpr int kThreads = TBMAX * TBMAX / kPerThread;
   int order = 0;
   while ((int64_t(1) << order) < nside) ++order;
-  const int tb = nside < kMaxTile ? (int)nside : kMaxTile;
+  const int tb = nside < TBMAX ? (int)nside : TBMAX;
   int log_tb = 0;
   while ((1 << log_tb) < tb) ++log_tb;
   const int tiles_per_face = (int)((nside / tb) * (nside / tb));
   const int ntiles = 12 * tiles_per_face;
   // Enough work-groups to fill 256 CUs several times over.
-  int64_t chunks = (2048 + ntiles - 1) / ntiles;
+  const int64_t want = TBMAX == 64 ? 2048 : 768;
+  int64_t chunks = (want + ntiles - 1) / ntiles;
   if (chunks > nrows) chunks = nrows;
   if (chunks < 1) chunks = 1;
   const int rows_per_chunk = (int)((nrows + chunks - 1) / chunks);
   chunks = (nrows + rows_per_chunk - 1) / rows_per_chunk;
   dim3 grid(ntiles, (unsigned)chunks);
-  remap_tiles<T><<<grid, kThreads, 0, stream>>>(
+  remap_tiles<T, TBMAX, NT><<<grid, kThreads, 0, stream>>>(
       static_cast<const T*>(in), static_cast<T*>(out), nside, order, tb, log_tb,
       tiles_per_face, nrows, rows_per_chunk, src, dst);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
+}
+
+template <typename T>
+int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Side dst,
+           hipStream_t stream) {
+  const bool ring = src.scheme == kRing || dst.scheme == kRing;
+  const bool nest_flat = (src.scheme == kNest && dst.scheme == kFlat) || (src.scheme == kFlat && dst.scheme == kNest);
+  if constexpr (sizeof(T) <= 4) {
+    if (ring && nside >= 128) return launch_tiles<T, 128, false>(in, out, nside, nrows, src, dst, stream);
+  }
+  if (nest_flat) return launch_tiles<T, 64, true>(in, out, nside, nrows, src, dst, stream);
+  return launch_tiles<T, 64, false>(in, out, nside, nrows, src, dst, stream);
 }
 
 }  // namespace

@@ -89,6 +89,7 @@ void free_plan(ShtPlan* p) {
   hipFree(p->d_bhat);
   hipFree(p->d_twiddle);
   hipFree(p->d_p);
+  hipFree(p->d_pt);
   hipFree(p->d_p_off);
   hipFree(p->d_nl);
   hipFree(p->d_fwd_items);
@@ -247,6 +248,8 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   p->at_rows = pfloats / p->kpad;
   double* d_ct = nullptr;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_p), std::max<int64_t>(pfloats, 1) * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void**>(&p->d_pt), std::max<int64_t>(pfloats, 1) * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_ct), ct.size() * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(d_ct, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -254,7 +257,7 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     free_plan(p);
     HPX_FAIL(HPX_ERR_HIP, "plan allocation failed: %s", hipGetErrorString(e));
   }
-  bytes += pfloats * (int64_t)sizeof(float);
+  bytes += 2 * pfloats * (int64_t)sizeof(float);
   rc = launch_legendre_table(*p, d_ct, stream);
   if (rc == HPX_OK) {
     e = hipStreamSynchronize(stream);

@@ -63,6 +63,7 @@ struct ShtPlan {
   float2* d_bhat = nullptr;
   float2* d_twiddle = nullptr;  // [max_M] exp(-2 pi i x / max_M)
   float* d_p = nullptr;         // Legendre table [m][parity][row][kpad]
+  float* d_pt = nullptr;        // the same transposed per (m, parity): [m][parity][k][row]
   int64_t* d_p_off = nullptr;   // [mmax + 1]
   int* d_nl = nullptr;          // [mmax * 2]
   int2* d_fwd_items = nullptr;  // forward GEMM work items (m, row block)
@@ -114,8 +115,10 @@ inline int64_t col_pad(int64_t nbatch) { return ((2 * nbatch + 31) / 32) * 32; }
 inline int64_t x_floats(const ShtPlan& p, int64_t nbatch) {
   return (int64_t)p.mmax * 2 * p.kpad * col_pad(nbatch);
 }
+// Rows after the last row of aT that the inverse GEMM may read (zero).
+constexpr int kAtSlackRows = 64;
 inline int64_t workspace_floats(const ShtPlan& p, int64_t nbatch) {
-  return x_floats(p, nbatch) + p.at_rows * col_pad(nbatch);
+  return x_floats(p, nbatch) + (p.at_rows + kAtSlackRows) * col_pad(nbatch);
 }
 
 // Launchers (return HPX_OK or an error code after setting the error text).
