@@ -199,15 +199,21 @@ __device__ __forceinline__ void wave_fft(float2* buf, int log2M, const float2* _
 }
 
 // DFT of length N of the wave's sequence (entered as z_j, or z_j * chirp_j
-// zero-padded to M for Bluestein).  Afterwards the DFT value at x < N is
-// dft_value(buf, ...).  Bluestein: X[m] = chirp_m conj(c_m),
-// c = FFT(conj(FFT(a) * bhat)) with bhat = FFT(conj chirp) / M.
+// zero-padded to M for Bluestein), in place: afterwards point x < N of the
+// sequence holds the DFT value at x.  Bluestein: X[x] = chirp_x conj(c_x),
+// c = FFT(conj(FFT(a) * bhat)) with bhat = FFT(conj chirp) / M; the chirp
+// factors (N <= M/2 of them, lane-contiguous) are loaded before the second
+// FFT for the classes up to 2048 points, so their latency hides behind it.
 template <int LO, int HI>
 __device__ __forceinline__ void wave_ring_dft(float2* buf, const RingPair& pr,
                                               const float2* __restrict__ bhat,
+                                              const float2* __restrict__ chirp,
                                               const float2* __restrict__ tw, int log2_tab, int lane) {
   wave_fft<LO, HI>(buf, pr.log2M, tw, log2_tab, lane);
   if (pr.M != pr.nphi) {
+    constexpr int kItC = ((1 << HI) / 2 + 63) / 64;
+    constexpr bool kEarly = HI <= 11;
+    float2 cw[kEarly ? kItC : 1];
     // all loads of the kernel spectrum in flight before the LDS updates
     constexpr int kIt = ((1 << HI) + 63) / 64;
     float2 bh[kIt];
@@ -222,8 +228,48 @@ __device__ __forceinline__ void wave_ring_dft(float2* buf, const RingPair& pr,
       const int px = pad(lane) + 68 * i;
       if (x < pr.M) buf[px] = cconj(cmul(buf[px], bh[i]));
     }
+    if constexpr (kEarly) {
+#pragma unroll
+      for (int i = 0; i < kItC; ++i) {
+        const int x = lane + 64 * i;
+        if (x < pr.nphi) cw[i] = chirp[x];
+      }
+    }
     wave_sync();
     wave_fft<LO, HI>(buf, pr.log2M, tw, log2_tab, lane);
+#pragma unroll
+    for (int i = 0; i < kItC; ++i) {
+      const int x = lane + 64 * i;
+      const int px = pad(lane) + 68 * i;
+      if (x < pr.nphi) {
+        const float2 w = kEarly ? cw[kEarly ? i : 0] : chirp[x];
+        buf[px] = cmul(w, cconj(buf[px]));
+      }
+    }
+    wave_sync();
+  }
+}
+
+// seq[x] *= chirp[x] for x < N (Bluestein pre-multiply of a packed sequence).
+template <int HI>
+__device__ __forceinline__ void chirp_premultiply(float2* buf, const float2* __restrict__ chirp, int N,
+                                                  int lane) {
+  constexpr int kItC = ((1 << HI) / 2 + 63) / 64;
+  constexpr int kB = kItC < 8 ? kItC : 8;
+#pragma unroll
+  for (int i0 = 0; i0 < kItC; i0 += kB) {
+    float2 w[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int x = lane + 64 * (i0 + u);
+      if (x < N) w[u] = chirp[x];
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int x = lane + 64 * (i0 + u);
+      const int px = pad(lane) + 68 * (i0 + u);
+      if (x < N) buf[px] = cmul(buf[px], w[u]);
+    }
   }
 }
 
@@ -311,6 +357,12 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
   const int64_t b = b0 + wave;
   const int stride = wave_stride(M);
   float2* buf = lds + wave * stride;
+  // the pair's phase factors exp(-i m phi0), staged once for the store loop
+  float2* phs = lds + WAVES * wave_stride(1 << HI);
+  {
+    const float2* ph = a.phase + (int64_t)k * a.mmax;
+    for (int m = threadIdx.x; m < a.mmax; m += kThreadsW) phs[m] = ph[m];
+  }
   if (b < a.nbatch) {
     const float* fn = f + b * a.npix + pr.off_n;
     const float* fs = f + b * a.npix + (pr.single ? pr.off_n : pr.off_s);
@@ -341,52 +393,27 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
       }
     }
     wave_sync();
-    wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab, lane);
+    wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
   }
   const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
-  const float2* ph = a.phase + (int64_t)k * a.mmax;
+  __syncthreads();
   if (a.layout == 0) {
     // folded spectra X[m][parity][k][2b + re/im]: fields fastest so that
     // consecutive lanes write consecutive columns
-    __syncthreads();
     const int64_t pstride = (int64_t)a.kpad * a.ncol;
     const int total = a.mmax * WAVES;
-    constexpr int kBatch = 4;
-    for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * kThreadsW) {
-      // loads of this thread's next kBatch elements first (phase and chirp
-      // tables), then the arithmetic and stores
-      float2 phv[kBatch], cm[kBatch], cc[kBatch];
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int e = e0 + u * kThreadsW;
-        const int m = e / WAVES;
-        if (e < total && m < K) {
-          phv[u] = ph[m];
-          if (blue) {
-            cm[u] = chirp[m];
-            cc[u] = chirp[m == 0 ? 0 : N - m];
-          }
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int e = e0 + u * kThreadsW;
-        if (e >= total) continue;
-        const int m = e / WAVES;
-        const int g = e - m * WAVES;
-        if (b0 + g >= a.nbatch) continue;
+    for (int e = threadIdx.x; e < total; e += kThreadsW) {
+      const int m = e / WAVES;
+      const int g = e - m * WAVES;
+      if (b0 + g >= a.nbatch) continue;
+      {
         float2 S = make_float2(0.f, 0.f), D = S;
         if (m < K) {
           const float2* seq = lds + g * stride;
-          float2 zm = seq[pad(m)], zc = seq[pad(m == 0 ? 0 : N - m)];
-          if (blue) {
-            zm = cmul(cm[u], cconj(zm));
-            zc = cmul(cc[u], cconj(zc));
-          }
           const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
           float2 Fn, Fs;
-          unpack_pair(zm, zc, phv[u], c, Fn, Fs);
+          unpack_pair(seq[pad(m)], seq[pad(m == 0 ? 0 : N - m)], phs[m], c, Fn, Fs);
           if (pr.single) {
             S = Fn;
             D = Fn;
@@ -407,8 +434,7 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
       float2 Fn = make_float2(0.f, 0.f), Fs = Fn;
       if (m < K) {
         const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
-        unpack_pair(dft_value<true>(buf, pr, m, chirp), dft_value<true>(buf, pr, m == 0 ? 0 : N - m, chirp),
-                    ph[m], c, Fn, Fs);
+        unpack_pair(buf[pad(m)], buf[pad(m == 0 ? 0 : N - m)], phs[m], c, Fn, Fs);
       }
       out[(int64_t)k * a.mmax + m] = Fn;
       if (!pr.single) out[(int64_t)(a.nring - 1 - k) * a.mmax + m] = Fs;
@@ -433,17 +459,23 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
   const int64_t b = b0 + wave;
   const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
-  const float2* ph = a.phase + (int64_t)k * a.mmax;
+  // the pair's phase factors exp(-i m phi0), staged once for the packing
+  float2* phs = lds + WAVES * wave_stride(1 << HI);
+  {
+    const float2* ph = a.phase + (int64_t)k * a.mmax;
+    for (int m = threadIdx.x; m < a.mmax; m += kThreadsW) phs[m] = ph[m];
+  }
 
   const int stride = wave_stride(M);
   for (int idx = threadIdx.x; idx < WAVES * stride; idx += kThreadsW) lds[idx] = make_float2(0.f, 0.f);
   __syncthreads();
+  // the packing leaves the chirp factors to a per-wave pass below
   if (a.layout == 0) {
     const int64_t pstride = (int64_t)a.kpad * a.ncol;
     const int total = K * WAVES;
     constexpr int kBatch = 4;
     for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * kThreadsW) {
-      float2 S[kBatch], D[kBatch], phv[kBatch];
+      float2 S[kBatch], D[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         const int e = e0 + u * kThreadsW;
@@ -453,7 +485,6 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
           const float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * (b0 + g);
           S[u] = *reinterpret_cast<const float2*>(base);
           D[u] = *reinterpret_cast<const float2*>(base + pstride);
-          phv[u] = ph[m];
         }
       }
 #pragma unroll
@@ -462,8 +493,8 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
         const int m = e / WAVES;
         const int g = e - m * WAVES;
         if (e < total && b0 + g < a.nbatch)
-          pack_pair<true>(lds + g * stride, pr, m, cadd(S[u], D[u]), csub(S[u], D[u]), cconj(phv[u]),
-                          wscale, a.cmode, chirp);
+          pack_pair<true>(lds + g * stride, pr, m, cadd(S[u], D[u]), csub(S[u], D[u]), cconj(phs[m]),
+                          wscale, a.cmode, nullptr);
       }
     }
   } else {
@@ -474,36 +505,26 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
       const float2* in = reinterpret_cast<const float2*>(X) + (b0 + g) * a.nring * (int64_t)a.mmax;
       const float2 Gn = in[(int64_t)k * a.mmax + m];
       const float2 Gs = pr.single ? Gn : in[(int64_t)(a.nring - 1 - k) * a.mmax + m];
-      pack_pair<true>(lds + g * stride, pr, m, Gn, Gs, cconj(ph[m]), wscale, a.cmode, chirp);
+      pack_pair<true>(lds + g * stride, pr, m, Gn, Gs, cconj(phs[m]), wscale, a.cmode, nullptr);
     }
   }
   __syncthreads();
   if (b >= a.nbatch) return;
   float2* buf = lds + wave * stride;
-  wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab, lane);
+  if (M != N) {
+    chirp_premultiply<HI>(buf, chirp, N, lane);
+    wave_sync();
+  }
+  wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
   float* fr = f + b * a.npix;
   constexpr int kIt = ((1 << HI) + 63) / 64;
-  constexpr int kBatch = kIt < 8 ? kIt : 8;
-  const bool blue = M != N;
 #pragma unroll
-  for (int i0 = 0; i0 < kIt; i0 += kBatch) {
-    float2 c[kBatch];
-    if (blue) {
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int j = lane + 64 * (i0 + u);
-        if (j < N) c[u] = chirp[j];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kBatch; ++u) {
-      const int j = lane + 64 * (i0 + u);
-      if (j < N) {
-        float2 r = buf[pad(lane) + 68 * (i0 + u)];
-        if (blue) r = cmul(c[u], cconj(r));
-        fr[pr.off_n + j] = r.x;
-        if (!pr.single) fr[pr.off_s + j] = -r.y;
-      }
+  for (int u = 0; u < kIt; ++u) {
+    const int j = lane + 64 * u;
+    if (j < N) {
+      const float2 r = buf[pad(lane) + 68 * u];
+      fr[pr.off_n + j] = r.x;
+      if (!pr.single) fr[pr.off_s + j] = -r.y;
     }
   }
 }
@@ -747,7 +768,8 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, const fl
   const int64_t nfg = (nbatch + waves - 1) / waves;
   const int64_t items = (int64_t)n * nfg;
   if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  const size_t lds_bytes = (size_t)waves * wave_stride(1 << (10 + c)) * sizeof(float2);
+  const size_t lds_bytes = ((size_t)waves * wave_stride(1 << (10 + c)) + p.mmax) * sizeof(float2);
+  if (lds_bytes > 160 * 1024) HPX_FAIL(HPX_ERR_UNSUPPORTED, "mmax too large for the ring FFT kernels");
   const unsigned grid = (unsigned)items;
   switch (c * 16 + waves) {
     case 0 * 16 + 4: launch_wave<4, 0>(analysis, grid, lds_bytes, in, out, a, s); break;

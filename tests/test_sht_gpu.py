@@ -214,3 +214,29 @@ def test_errors():
         sht(torch.zeros(192, dtype=torch.float64, device="cuda"))
     with pytest.raises(RuntimeError):
         sht(torch.zeros(191, device="cuda"))
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_nside1024_all_fft_classes(precision):
+    """nside 1024 (rings of 4..4096 pixels: every FFT size class, including the
+    8192-point Bluestein rings of the work-group-wide kernel), lmax = mmax =
+    129, 2 fields, against the streamed FP64 oracle."""
+    nside, L = 1024, 129
+    o = S.SHTOracle.healpix(nside, L, L, streamed=True)
+    alm = band_limited_alm(L, L, 2, 7)
+    f64 = o.inverse(alm)
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=L, precision=precision)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=L, precision=precision)
+    f32 = f64.astype(np.float32)
+    a = sht(torch.from_numpy(f32).cuda()).cpu().numpy()
+    err_f = rel_l2(a, o.forward(f32.astype(np.float64)))
+    back = isht(torch.from_numpy(alm.astype(np.complex64)).cuda()).cpu().numpy()
+    err_i = rel_l2(back, o.inverse(alm.astype(np.complex64).astype(np.complex128)))
+    print(f"[nside1024 {precision}] forward rel-L2 {err_f:.3e}  inverse rel-L2 {err_i:.3e}", flush=True)
+    assert err_f < TOL[precision] and err_i < TOL[precision]
+
+
+def test_unsupported_nside_is_a_clean_error():
+    """nside 2048 needs 16384-point Bluestein FFTs: plan creation refuses it."""
+    with pytest.raises(NotImplementedError):
+        ShtPlan.healpix(2048, 9, 9)
