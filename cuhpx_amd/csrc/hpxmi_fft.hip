@@ -367,7 +367,8 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
     const float* fn = f + b * a.npix + pr.off_n;
     const float* fs = f + b * a.npix + (pr.single ? pr.off_n : pr.off_s);
     constexpr int kIt = ((1 << HI) + 63) / 64;
-    constexpr int kBatch = kIt < 8 ? kIt : 8;
+    // all ring samples of the field in flight at once (one round trip)
+    constexpr int kBatch = kIt < 32 ? kIt : 32;
 #pragma unroll
     for (int i0 = 0; i0 < kIt; i0 += kBatch) {
       float2 z[kBatch], c[kBatch];
