@@ -460,6 +460,28 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
   const int64_t b = b0 + wave;
   const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
+  // folded spectra (S, D) of element e = (m = e / WAVES, field g = e % WAVES),
+  // kBatch elements per thread per batch; the first batch is requested
+  // before the LDS set-up and each next batch while the current one is packed
+  const int64_t pstride = (int64_t)a.kpad * a.ncol;
+  const int total = K * WAVES;
+  constexpr int kBatch = 4;
+  float2 S[kBatch], D[kBatch];
+  auto load_batch = [&](int e0) {
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const int e = e0 + u * kThreadsW;
+      const int m = e / WAVES;
+      const int g = e - m * WAVES;
+      if (e < total && b0 + g < a.nbatch) {
+        const float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * (b0 + g);
+        S[u] = *reinterpret_cast<const float2*>(base);
+        D[u] = *reinterpret_cast<const float2*>(base + pstride);
+      }
+    }
+  };
+  if (a.layout == 0) load_batch(threadIdx.x);
+
   // the pair's phase factors exp(-i m phi0), staged once for the packing
   float2* phs = lds + WAVES * wave_stride(1 << HI);
   {
@@ -472,29 +494,21 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
   __syncthreads();
   // the packing leaves the chirp factors to a per-wave pass below
   if (a.layout == 0) {
-    const int64_t pstride = (int64_t)a.kpad * a.ncol;
-    const int total = K * WAVES;
-    constexpr int kBatch = 4;
     for (int e0 = threadIdx.x; e0 < total; e0 += kBatch * kThreadsW) {
-      float2 S[kBatch], D[kBatch];
+      float2 Sc[kBatch], Dc[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        const int e = e0 + u * kThreadsW;
-        const int m = e / WAVES;
-        const int g = e - m * WAVES;
-        if (e < total && b0 + g < a.nbatch) {
-          const float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * (b0 + g);
-          S[u] = *reinterpret_cast<const float2*>(base);
-          D[u] = *reinterpret_cast<const float2*>(base + pstride);
-        }
+        Sc[u] = S[u];
+        Dc[u] = D[u];
       }
+      if (e0 + kBatch * kThreadsW < total) load_batch(e0 + kBatch * kThreadsW);
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
         const int e = e0 + u * kThreadsW;
         const int m = e / WAVES;
         const int g = e - m * WAVES;
         if (e < total && b0 + g < a.nbatch)
-          pack_pair<true>(lds + g * stride, pr, m, cadd(S[u], D[u]), csub(S[u], D[u]), cconj(phs[m]),
+          pack_pair<true>(lds + g * stride, pr, m, cadd(Sc[u], Dc[u]), csub(Sc[u], Dc[u]), cconj(phs[m]),
                           wscale, a.cmode, nullptr);
       }
     }
