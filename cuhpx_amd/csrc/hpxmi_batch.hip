@@ -103,6 +103,7 @@ int autotune(ShtPlan& p, int precision, int64_t nbatch, float* ws, int64_t ws_by
                               : launch_fft_class(p, false, c, w, X, t.f, nbatch, 0, 1.f, 0, s, 0);
             },
             s, t.e0, t.e1, t.flush, &ms);
+        if (rc == HPX_ERR_UNSUPPORTED) continue;  // shape does not fit this plan's LDS
         if (rc != HPX_OK) return rc;
         if (ms < best) {
           best = ms;

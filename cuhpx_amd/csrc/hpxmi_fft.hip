@@ -778,7 +778,7 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, const fl
     return HPX_OK;
   }
   // class c < 3: FFT length <= 2^(10 + c); one field per wavefront
-  if (waves != kFftWaveChoices[c][0] && waves != kFftWaveChoices[c][1])
+  if (waves != kFftWaveChoices[c][0] && waves != kFftWaveChoices[c][1] && waves != kFftWaveChoices[c][2])
     HPX_FAIL(HPX_ERR_ARG, "no FFT kernel with %d waves for size class %d", waves, c);
   const int64_t nfg = (nbatch + waves - 1) / waves;
   const int64_t items = (int64_t)n * nfg;
@@ -786,13 +786,16 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, const fl
   const size_t lds_bytes = ((size_t)waves * wave_stride(1 << (10 + c)) + p.mmax) * sizeof(float2);
   if (lds_bytes > 160 * 1024) HPX_FAIL(HPX_ERR_UNSUPPORTED, "mmax too large for the ring FFT kernels");
   const unsigned grid = (unsigned)items;
-  switch (c * 16 + waves) {
-    case 0 * 16 + 4: launch_wave<4, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 0 * 16 + 8: launch_wave<8, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 1 * 16 + 2: launch_wave<2, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 1 * 16 + 4: launch_wave<4, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 2 * 16 + 1: launch_wave<1, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
-    default: launch_wave<2, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
+  switch (c * 32 + waves) {
+    case 0 * 32 + 4: launch_wave<4, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 0 * 32 + 8: launch_wave<8, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 0 * 32 + 16: launch_wave<16, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 1 * 32 + 2: launch_wave<2, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 1 * 32 + 4: launch_wave<4, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 1 * 32 + 8: launch_wave<8, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 2 * 32 + 1: launch_wave<1, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
+    case 2 * 32 + 2: launch_wave<2, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
+    default: launch_wave<4, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
   }
   HPX_LAUNCH_CHECK();
   return HPX_OK;

@@ -23,7 +23,7 @@ struct GemmTune {
   int inv_ct = 2;
 };
 // Allowed values (what the kernels are instantiated for).
-constexpr int kFftWaveChoices[3][2] = {{4, 8}, {2, 4}, {1, 2}};
+constexpr int kFftWaveChoices[3][3] = {{4, 8, 16}, {2, 4, 8}, {1, 2, 4}};
 constexpr int kGemmCtChoices[2] = {1, 2};
 
 // One mirror pair of rings (north t, south nring-1-t); the equator of a grid
