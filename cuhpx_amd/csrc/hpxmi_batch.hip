@@ -121,11 +121,15 @@ int autotune(ShtPlan& p, int precision, int64_t nbatch, float* ws, int64_t ws_by
       best_f = ms;
       gt.fwd_ct = ct;
     }
-    rc = time_ms([&] { return launch_gemm_inv(p, precision, ct, aT, X, nbatch, s); }, s, t.e0, t.e1, t.flush, &ms);
-    if (rc != HPX_OK) return rc;
-    if (ms < best_i) {
-      best_i = ms;
-      gt.inv_ct = ct;
+    for (int rh : kGemmRhChoices) {
+      rc = time_ms([&] { return launch_gemm_inv(p, precision, ct, rh, aT, X, nbatch, s); }, s, t.e0, t.e1, t.flush,
+                   &ms);
+      if (rc != HPX_OK) return rc;
+      if (ms < best_i) {
+        best_i = ms;
+        gt.inv_ct = ct;
+        gt.inv_rh = rh;
+      }
     }
   }
   HPX_HIP(hipStreamSynchronize(s));
@@ -198,6 +202,7 @@ extern "C" int hpx_sht_tuned_shapes(const hpx_sht_plan* plan, int precision, int
     for (int c = 0; c < 3; ++c) shapes[3 * d + c] = f.waves[d][c];
   shapes[6] = g.fwd_ct;
   shapes[7] = g.inv_ct;
+  shapes[8] = g.inv_rh;
   return HPX_OK;
 }
 

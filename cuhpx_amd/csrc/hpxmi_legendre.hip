@@ -125,24 +125,29 @@ __global__ __launch_bounds__(kThreads) void legendre_table_kernel(
 // Forward: aT[m][p][i][col] = sum_k P[m][p][i][k] X[m][p][k][col]
 // ---------------------------------------------------------------------------
 
+// Double-buffered table tile of the GEMMs: ROWS rows of a kKTile chunk, per
+// parity, FP32 or split into BF16 hi / lo.
+template <int PREC, int ROWS>
+struct GemmLds;
+
+template <int ROWS>
+struct GemmLds<HPX_PREC_FP32, ROWS> {
+  float a[2][ROWS][kKTile + kLdsPadF];
+};
+
+template <int ROWS>
+struct GemmLds<HPX_PREC_BF16X3, ROWS> {
+  __bf16 hi[2][ROWS][kKTile + kLdsPadH];
+  __bf16 lo[2][ROWS][kKTile + kLdsPadH];
+};
+
+template <int ROWS>
+struct GemmLds<HPX_PREC_BF16, ROWS> {
+  __bf16 hi[2][ROWS][kKTile + kLdsPadH];
+};
+
 template <int PREC>
-struct FwdLds;
-
-template <>
-struct FwdLds<HPX_PREC_FP32> {
-  float a[2][kRowsPerParity][kKTile + kLdsPadF];
-};
-
-template <>
-struct FwdLds<HPX_PREC_BF16X3> {
-  __bf16 hi[2][kRowsPerParity][kKTile + kLdsPadH];
-  __bf16 lo[2][kRowsPerParity][kKTile + kLdsPadH];
-};
-
-template <>
-struct FwdLds<HPX_PREC_BF16> {
-  __bf16 hi[2][kRowsPerParity][kKTile + kLdsPadH];
-};
+using FwdLds = GemmLds<PREC, kRowsPerParity>;
 
 // Split a value into BF16 hi and lo (x = hi + lo to ~16 bits more).
 template <int PREC>
@@ -166,13 +171,13 @@ __device__ __forceinline__ void split_to(const float (&v)[8], bf16x8& hi, bf16x8
 // rows multiply zeros.
 template <int PREC, int CT, class LdsT>
 __device__ __forceinline__ void mfma_chunk(const LdsT& L, const float (&bv)[2][CT][8], floatx4 (&acc)[2][4][CT],
-                                           int lane) {
+                                           int lane, int row_off = 0) {
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
     if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt) {
-        const int ar = rt * 16 + (lane & 15);
+        const int ar = row_off + rt * 16 + (lane & 15);
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           const float av = L.a[p][ar][4 * q + (lane >> 4)];
@@ -187,7 +192,7 @@ __device__ __forceinline__ void mfma_chunk(const LdsT& L, const float (&bv)[2][C
       for (int ct = 0; ct < CT; ++ct) split_to<PREC>(bv[p][ct], bhi[ct], blo[ct]);
 #pragma unroll
       for (int rt = 0; rt < 4; ++rt) {
-        const int ar = rt * 16 + (lane & 15);
+        const int ar = row_off + rt * 16 + (lane & 15);
         const bf16x8 ahi = *reinterpret_cast<const bf16x8*>(&L.hi[p][ar][8 * (lane >> 4)]);
         bf16x8 alo;
         if constexpr (PREC == HPX_PREC_BF16X3)
@@ -240,11 +245,14 @@ __device__ __forceinline__ float buffer_load(__amdgpu_buffer_rsrc_t r, int voff_
 // the main loop); row(p, r): destination of local row r of parity p, or
 // nullptr; only the first ncol_left columns of a row exist.  All threads of
 // the work-group call it.
-template <int CT, class RowFn>
+template <int CT, int RH = 1, class RowFn>
 __device__ __forceinline__ void store_tiles_lds(const floatx4 (&acc)[2][4][CT], float* T, int wave, int lane,
                                                 int64_t ncol_left, RowFn row) {
+  // RH row halves of 64 rows; wave = 4 * row half + column wave
   constexpr int kW = 64 * CT;
   constexpr int kS = kW + 4;
+  const int rh = wave >> 2;
+  const int cw = wave & 3;
 #pragma unroll
   for (int p = 0; p < 2; ++p) {
     __syncthreads();
@@ -254,9 +262,10 @@ __device__ __forceinline__ void store_tiles_lds(const floatx4 (&acc)[2][4][CT], 
       for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          T[(rt * 16 + 4 * (lane >> 4) + r) * kS + wave * 16 * CT + ct * 16 + (lane & 15)] = acc[p][rt][ct][r];
+          T[(rh * 64 + rt * 16 + 4 * (lane >> 4) + r) * kS + cw * 16 * CT + ct * 16 + (lane & 15)] =
+              acc[p][rt][ct][r];
     __syncthreads();
-    for (int v = threadIdx.x; v < 64 * (kW / 4); v += kThreads) {
+    for (int v = threadIdx.x; v < 64 * RH * (kW / 4); v += kThreads * RH) {
       const int r = v / (kW / 4);
       const int c4 = v - r * (kW / 4);
       float* dst = row(p, r);
@@ -398,19 +407,25 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
 // Inverse: Y[m][p][k][col] = sum_i P[m][p][i][k] aT[m][p][i][col]
 // ---------------------------------------------------------------------------
 
-template <int PREC, int CT>
-__global__ __launch_bounds__(kThreads, CT == 1 ? 4 : 2) void legendre_inv_kernel(
+// RH: row halves of 64 ring pairs per work-group (4 * RH waves: wave = 4 *
+// row half + column wave).  With RH = 2 a work-group covers 128 ring pairs,
+// so each coefficient chunk serves both halves and the per-work-group set-up
+// and output staging are spread over twice the work.
+template <int PREC, int CT, int RH>
+__global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_kernel(
     const float* __restrict__ aT, float* __restrict__ X, const float* __restrict__ PT,
     const int64_t* __restrict__ p_off, const int* __restrict__ nl, int nkb, int nitems, int ncb, int kpad,
     int64_t ncol, int64_t at_rows) {
   // same tiling as the forward kernel with the roles of i and k exchanged:
-  // rows = 64 ring pairs, reduction over the rows i of aT, A tiles from the
+  // rows = ring pairs, reduction over the rows i of aT, A tiles from the
   // transposed table PT[m][p][k][i]
   // table tiles (double buffer) during the main loop, the output tile after it
-  constexpr size_t kTileBytes = sizeof(float) * 64 * (64 * CT + 4);
-  constexpr size_t kLdsBytes = 2 * sizeof(FwdLds<PREC>) > kTileBytes ? 2 * sizeof(FwdLds<PREC>) : kTileBytes;
+  constexpr int kRows = 64 * RH;
+  using Lds = GemmLds<PREC, kRows>;
+  constexpr size_t kTileBytes = sizeof(float) * kRows * (64 * CT + 4);
+  constexpr size_t kLdsBytes = 2 * sizeof(Lds) > kTileBytes ? 2 * sizeof(Lds) : kTileBytes;
   __shared__ __align__(16) unsigned char smem[kLdsBytes];
-  FwdLds<PREC>* lds = reinterpret_cast<FwdLds<PREC>*>(smem);
+  Lds* lds = reinterpret_cast<Lds*>(smem);
   // m outer, ring-pair block inner: the nkb work-groups of one m read the same
   // coefficient rows aT[m] and land on one XCD (contiguous item ranges)
   const int idx = xcd_local_index(blockIdx.x, gridDim.x);
@@ -421,12 +436,13 @@ __global__ __launch_bounds__(kThreads, CT == 1 ? 4 : 2) void legendre_inv_kernel
   const int kb = item - m * nkb;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int col0 = cb * (64 * CT) + wave * (16 * CT);
+  const int rh = wave >> 2;
+  const int col0 = cb * (64 * CT) + (wave & 3) * (16 * CT);
   const bool wave_active = col0 < ncol;
   const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
   const int padp[2] = {(nlp[0] + kRowPad - 1) / kRowPad * kRowPad, (nlp[1] + kRowPad - 1) / kRowPad * kRowPad};
   const int nmax = nlp[0] > nlp[1] ? nlp[0] : nlp[1];
-  const int k0 = kb * kKBlock;
+  const int k0 = kb * kRows;
   const int nc = (int)ncol;
 
   floatx4 acc[2][4][CT];
@@ -437,9 +453,9 @@ __global__ __launch_bounds__(kThreads, CT == 1 ? 4 : 2) void legendre_inv_kernel
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct) acc[p][rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  // staging: 2 parities x 64 ring pairs x 2 halves of 16 i
-  const int sp = threadIdx.x >> 7;
-  const int sr = (threadIdx.x >> 1) & 63;
+  // staging: 2 parities x kRows ring pairs x 2 halves of 16 i
+  const int sp = threadIdx.x / (2 * kRows);
+  const int sr = (threadIdx.x >> 1) & (kRows - 1);
   const int sh = (threadIdx.x & 1) * 16;
   const int spad = padp[sp];
   const float* Pbase = PT + p_off[m] + (sp ? (int64_t)padp[0] * kpad : 0) + (int64_t)sr * spad + sh;
@@ -464,7 +480,7 @@ __global__ __launch_bounds__(kThreads, CT == 1 ? 4 : 2) void legendre_inv_kernel
       for (int q = 0; q < 16; ++q) v[q] = 0.f;
     }
   };
-  auto store_a = [&](FwdLds<PREC>& L, const float* v) {
+  auto store_a = [&](Lds& L, const float* v) {
     if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) L.a[sp][sr][sh + q] = v[q];
@@ -476,8 +492,6 @@ __global__ __launch_bounds__(kThreads, CT == 1 ? 4 : 2) void legendre_inv_kernel
       for (int q = 0; q < 16; ++q) L.hi[sp][sr][sh + q] = (__bf16)v[q];
     }
   };
-  // B fragments: rows i of aT (this m's rows of parity p); rows at or beyond
-  // the parity's row count hold other data, so they enter as zero
   // B fragments: rows i of aT (this m's rows of parity p).  Rows beyond the
   // parity's count are zero padding, the other parity's or the next m's
   // rows, or the zero slack after aT: finite values that meet zero table
@@ -506,7 +520,7 @@ __global__ __launch_bounds__(kThreads, CT == 1 ? 4 : 2) void legendre_inv_kernel
         load_a((c + 1) * kKTile, pa);
         if (wave_active) load_b((c + 1) * kKTile, b1);
       }
-      if (wave_active) mfma_chunk<PREC, CT>(lds[0], b0, acc, lane);
+      if (wave_active) mfma_chunk<PREC, CT>(lds[0], b0, acc, lane, rh * 64);
       if (c + 1 < nchunk) store_a(lds[1], pa);
       __syncthreads();
       if (c + 1 >= nchunk) break;
@@ -514,16 +528,16 @@ __global__ __launch_bounds__(kThreads, CT == 1 ? 4 : 2) void legendre_inv_kernel
         load_a((c + 2) * kKTile, pa);
         if (wave_active) load_b((c + 2) * kKTile, b0);
       }
-      if (wave_active) mfma_chunk<PREC, CT>(lds[1], b1, acc, lane);
+      if (wave_active) mfma_chunk<PREC, CT>(lds[1], b1, acc, lane, rh * 64);
       if (c + 2 < nchunk) store_a(lds[0], pa);
       __syncthreads();
     }
   }
 
-  // rows k0..k0+63 of X[m][p], this work-group's columns
+  // rows k0..k0+kRows-1 of X[m][p], this work-group's columns
   const int64_t pstride = (int64_t)kpad * ncol;
   float* Xb = X + (int64_t)m * 2 * pstride + (int64_t)k0 * ncol + cb * (64 * CT);
-  store_tiles_lds<CT>(acc, reinterpret_cast<float*>(smem), wave, lane, ncol - cb * (64 * CT),
+  store_tiles_lds<CT, RH>(acc, reinterpret_cast<float*>(smem), wave, lane, ncol - cb * (64 * CT),
                       [&](int p, int r) -> float* {
                         return k0 + r < kpad ? Xb + p * pstride + (int64_t)r * ncol : nullptr;
                       });
@@ -736,25 +750,32 @@ int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, flo
   return HPX_OK;
 }
 
-int launch_gemm_inv(const ShtPlan& p, int precision, int ct, const float* aT, float* X,
+int launch_gemm_inv(const ShtPlan& p, int precision, int ct, int rh, const float* aT, float* X,
                     int64_t nbatch, hipStream_t s) {
   if (nbatch <= 0) return HPX_OK;
+  if (rh != 1 && rh != 2) HPX_FAIL(HPX_ERR_ARG, "no inverse GEMM with %d row halves", rh);
   const int64_t ncol = col_pad(nbatch);
   const int64_t ncb = (ncol + 64 * ct - 1) / (64 * ct);
-  const int nkb = (p.kpad + kKBlock - 1) / kKBlock;
+  const int nkb = (p.kpad + 64 * rh - 1) / (64 * rh);
   const int nitems = nkb * p.mmax;
   if ((int64_t)nitems * ncb > 0x7ffffff0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   const dim3 grid(xcd_grid((int64_t)nitems * ncb));
-#define HPX_INV(PR, CT)                                                                           \
-  legendre_inv_kernel<PR, CT><<<grid, kThreads, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, nkb, nitems, \
-                                                        (int)ncb, p.kpad, ncol, p.at_rows)
-  switch (precision * 4 + ct) {
-    case HPX_PREC_FP32 * 4 + 1: HPX_INV(HPX_PREC_FP32, 1); break;
-    case HPX_PREC_FP32 * 4 + 2: HPX_INV(HPX_PREC_FP32, 2); break;
-    case HPX_PREC_BF16X3 * 4 + 1: HPX_INV(HPX_PREC_BF16X3, 1); break;
-    case HPX_PREC_BF16X3 * 4 + 2: HPX_INV(HPX_PREC_BF16X3, 2); break;
-    case HPX_PREC_BF16 * 4 + 1: HPX_INV(HPX_PREC_BF16, 1); break;
-    case HPX_PREC_BF16 * 4 + 2: HPX_INV(HPX_PREC_BF16, 2); break;
+#define HPX_INV(PR, CT, RH)                                                                                  \
+  legendre_inv_kernel<PR, CT, RH><<<grid, kThreads * RH, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, nkb, nitems, \
+                                                                 (int)ncb, p.kpad, ncol, p.at_rows)
+  switch (precision * 8 + ct * 2 + rh - 1) {
+    case HPX_PREC_FP32 * 8 + 2: HPX_INV(HPX_PREC_FP32, 1, 1); break;
+    case HPX_PREC_FP32 * 8 + 3: HPX_INV(HPX_PREC_FP32, 1, 2); break;
+    case HPX_PREC_FP32 * 8 + 4: HPX_INV(HPX_PREC_FP32, 2, 1); break;
+    case HPX_PREC_FP32 * 8 + 5: HPX_INV(HPX_PREC_FP32, 2, 2); break;
+    case HPX_PREC_BF16X3 * 8 + 2: HPX_INV(HPX_PREC_BF16X3, 1, 1); break;
+    case HPX_PREC_BF16X3 * 8 + 3: HPX_INV(HPX_PREC_BF16X3, 1, 2); break;
+    case HPX_PREC_BF16X3 * 8 + 4: HPX_INV(HPX_PREC_BF16X3, 2, 1); break;
+    case HPX_PREC_BF16X3 * 8 + 5: HPX_INV(HPX_PREC_BF16X3, 2, 2); break;
+    case HPX_PREC_BF16 * 8 + 2: HPX_INV(HPX_PREC_BF16, 1, 1); break;
+    case HPX_PREC_BF16 * 8 + 3: HPX_INV(HPX_PREC_BF16, 1, 2); break;
+    case HPX_PREC_BF16 * 8 + 4: HPX_INV(HPX_PREC_BF16, 2, 1); break;
+    case HPX_PREC_BF16 * 8 + 5: HPX_INV(HPX_PREC_BF16, 2, 2); break;
     default: HPX_FAIL(HPX_ERR_ARG, "no inverse GEMM for precision %d, %d column tiles", precision, ct);
   }
 #undef HPX_INV
@@ -805,7 +826,7 @@ int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float
   HPX_LAUNCH_CHECK();
   int rc = launch_user_to_coeff(p, alm, aT, nbatch, s);
   if (rc != HPX_OK) return rc;
-  return launch_gemm_inv(p, precision, t.inv_ct, aT, ws, nbatch, s);
+  return launch_gemm_inv(p, precision, t.inv_ct, t.inv_rh, aT, ws, nbatch, s);
 }
 
 }  // namespace hpxmi

@@ -14,17 +14,19 @@ namespace hpxmi {
 // wavefront) per work-group for the wave-level size classes 0..2, for the
 // analysis [0] and synthesis [1] directions.
 struct FftTune {
-  int waves[2][3] = {{8, 4, 2}, {8, 4, 2}};
+  int waves[2][3] = {{16, 8, 2}, {16, 8, 2}};
 };
 // Launch shapes of the Legendre GEMMs for one (precision, column count):
 // 16-column MFMA tiles per wavefront (work-group column block = 64 * ct).
 struct GemmTune {
   int fwd_ct = 2;
   int inv_ct = 2;
+  int inv_rh = 1;  // inverse: 64-ring-pair row halves per work-group (1 or 2)
 };
 // Allowed values (what the kernels are instantiated for).
 constexpr int kFftWaveChoices[3][3] = {{4, 8, 16}, {2, 4, 8}, {1, 2, 4}};
 constexpr int kGemmCtChoices[2] = {1, 2};
+constexpr int kGemmRhChoices[2] = {1, 2};
 
 // One mirror pair of rings (north t, south nring-1-t); the equator of a grid
 // with an odd number of rings is a pair with north == south (single = 1).
@@ -136,7 +138,7 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int cls, int waves, const 
 // coefficient layout changes aT <-> a[b][l][m].
 int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, float* aT,
                     int64_t nbatch, hipStream_t s);
-int launch_gemm_inv(const ShtPlan& p, int precision, int ct, const float* aT, float* X,
+int launch_gemm_inv(const ShtPlan& p, int precision, int ct, int rh, const float* aT, float* X,
                     int64_t nbatch, hipStream_t s);
 int launch_coeff_to_user(const ShtPlan& p, const float* aT, float* alm, int64_t nbatch,
                          hipStream_t s);

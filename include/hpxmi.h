@@ -216,9 +216,10 @@ int hpx_sht_stage_ring_synthesis(const hpx_sht_plan* plan, float* f, int64_t nba
 int hpx_sht_autotune(hpx_sht_plan* plan, int precision, int64_t nbatch, void* workspace,
                      int64_t workspace_bytes, hipStream_t stream);
 
-/* The launch shapes used for (precision, nbatch): shapes[0..2] analysis
- * fields per work-group of FFT classes 0..2, shapes[3..5] the same for
- * synthesis, shapes[6] / shapes[7] forward / inverse GEMM column tiles. */
+/* The launch shapes used for (precision, nbatch), 9 values: shapes[0..2]
+ * analysis fields per work-group of FFT classes 0..2, shapes[3..5] the same
+ * for synthesis, shapes[6] / shapes[7] forward / inverse GEMM column tiles,
+ * shapes[8] inverse GEMM ring-pair rows per work-group in units of 64. */
 int hpx_sht_tuned_shapes(const hpx_sht_plan* plan, int precision, int64_t nbatch,
                          int32_t* shapes);
 
