@@ -427,13 +427,17 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
   __shared__ __align__(16) unsigned char smem[kLdsBytes];
   Lds* lds = reinterpret_cast<Lds*>(smem);
   // m outer, ring-pair block inner: the nkb work-groups of one m read the same
-  // coefficient rows aT[m] and land on one XCD (contiguous item ranges)
+  // coefficient rows aT[m] and land on one XCD (contiguous item ranges).  The
+  // reduction length of m is ~(mmax - m) / 2, so m runs in the order 0,
+  // mmax-1, 1, mmax-2, ...: every contiguous eighth of the items (one XCD's
+  // share) then carries the same total work.
   const int idx = xcd_local_index(blockIdx.x, gridDim.x);
   if (idx >= nitems * ncb) return;
   const int item = idx / ncb;
   const int cb = idx - item * ncb;
-  const int m = item / nkb;
-  const int kb = item - m * nkb;
+  const int mi = item / nkb;
+  const int kb = item - mi * nkb;
+  const int m = (mi & 1) ? nitems / nkb - 1 - (mi >> 1) : (mi >> 1);
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int rh = wave >> 2;

@@ -169,7 +169,10 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   p->max_M = max_M;
   p->log2_max_M = ilog2(max_M);
 
-  // ring pairs grouped by FFT size class
+  // Ring pairs grouped by FFT size class, in ring order.  (A cost-balanced
+  // order, pairs sorted by estimated FFT cost and dealt round-robin over the
+  // 8 XCDs' contiguous ranges, sped up the 4- and 8-field work-groups but
+  // slowed the 16-field ones that the tuner picks at nside 256.)
   std::vector<int> plist;
   for (int c = 0; c < kFftClasses; ++c) {
     p->class_off[c] = (int)plist.size();
