@@ -1,8 +1,9 @@
 """cuhpx_amd: MI355X-native HEALPix spherical harmonic transforms and remaps.
 
 Public surface mirrors the reference package's hot path (cuhpx/__init__.py:16-19):
-SHTCUDA, iSHTCUDA, Grid, Regridding, ring2nest, nest2ring, nest2flat,
-flat2nest, ring2flat, flat2ring, flat2flat and sht_tools, plus MixedSHT
+SHTCUDA, iSHTCUDA, VectorSHT, VectoriSHT, Grid, Regridding, ring2nest,
+nest2ring, nest2flat, flat2nest, ring2flat, flat2ring, flat2flat and
+sht_tools, plus MixedSHT
 (mixed-size requests through one HIP graph launch).  The device work is
 hand-written HIP for gfx950 in libhpxmi.so, reached through a C ABI
 (include/hpxmi.h).
@@ -12,7 +13,7 @@ from . import cuhpx_fft, sht_tools
 from .hpx_batch import MixedSHT
 from .hpx_regrid import Grid, InverseLatLonSHT, LatLonSHT, Regridding
 from .hpx_remap import flat2flat, flat2nest, flat2ring, nest2flat, nest2ring, ring2flat, ring2nest
-from .hpx_sht import SHTCUDA, ShtPlan, iSHTCUDA
+from .hpx_sht import SHTCUDA, ShtPlan, VectoriSHT, VectorSHT, iSHTCUDA
 
 __all__ = [
     "Grid",
@@ -22,6 +23,8 @@ __all__ = [
     "cuhpx_fft",
     "SHTCUDA",
     "iSHTCUDA",
+    "VectorSHT",
+    "VectoriSHT",
     "ShtPlan",
     "MixedSHT",
     "ring2nest",

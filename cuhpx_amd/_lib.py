@@ -60,6 +60,10 @@ SIGNATURES = {
     "hpx_sht_inverse": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_forward_adjoint": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_inverse_adjoint": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_vsht_plan_create_healpix": (_c_int, [_c_int, _c_int, _c_int, _dp, _c_int, _vp, ctypes.POINTER(_vp)]),
+    "hpx_vsht_workspace_bytes": (_c_i64, [_vp, _c_i64]),
+    "hpx_vsht_forward": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_vsht_inverse": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_ring_rfft": (_c_int, [_vp, _vp, _vp, _c_i64, _vp]),
     "hpx_ring_irfft": (_c_int, [_vp, _vp, _vp, _c_i64, _vp]),
     "hpx_sht_stage_ring_analysis": (_c_int, [_vp, _vp, _c_i64, _vp, _c_i64, _vp]),

@@ -122,6 +122,7 @@ int autotune(ShtPlan& p, int precision, int64_t nbatch, float* ws, int64_t ws_by
       gt.fwd_ct = ct;
     }
     for (int rh : kGemmRhChoices) {
+      if (p.vector && rh != 1) continue;  // vector GEMMs run 64 ring pairs per work-group
       rc = time_ms([&] { return launch_gemm_inv(p, precision, ct, rh, aT, X, nbatch, s); }, s, t.e0, t.e1, t.flush,
                    &ms);
       if (rc != HPX_OK) return rc;
@@ -239,6 +240,7 @@ extern "C" int hpx_sht_mixed_create(int nplans, const hpx_sht_plan* const* plans
   for (int i = 0; i < nplans; ++i) {
     int rc = validate(R(plans[i]), precision, nbatch[i], INT64_MAX);
     if (rc != HPX_OK) return rc;
+    if (R(plans[i])->vector) HPX_FAIL(HPX_ERR_ARG, "plan %d is a vector plan; mixed requests take scalar plans", i);
   }
 
   CaptureResources cr;

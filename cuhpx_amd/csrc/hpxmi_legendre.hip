@@ -121,6 +121,112 @@ __global__ __launch_bounds__(kThreads) void legendre_table_kernel(
   }
 }
 
+// P^mu_L(x) for L = mu, mu + 1, ... by the recurrence above, two consecutive
+// degrees at a time (v0 = P_L, v1 = P_{L+1}), carried as v * 2^(600 e).
+struct ScaledOrder {
+  double v0, v1;
+  int e, L, mu;
+
+  __device__ void init(int order, double x, double s) {
+    const double big = 0x1p600, tiny = 0x1p-600;
+    mu = order;
+    L = order;
+    e = 0;
+    double pmm = 0.28209479177387814347;  // 1 / sqrt(4 pi)
+    for (int i = 1; i <= order; ++i) {
+      pmm *= sqrt((2.0 * i + 1.0) / (2.0 * i)) * s;
+      if (pmm != 0.0 && fabs(pmm) < tiny) {
+        pmm *= big;
+        --e;
+      }
+    }
+    v0 = pmm;
+    v1 = sqrt(2.0 * order + 3.0) * x * pmm;
+    if (e < 0 && fabs(v1) > 1.0) {
+      v0 *= tiny;
+      v1 *= tiny;
+      ++e;
+    }
+  }
+  // P^mu_l for l in {L, L + 1}; zero below the order and while scaled down
+  __device__ double at(int l) const {
+    if (l < mu || e != 0) return 0.0;
+    return l == L ? v0 : v1;
+  }
+  __device__ void advance(double x) {
+    const double dl = L + 2, dm = mu;
+    const double a = sqrt((2 * dl - 1) / (dl - dm) * (2 * dl + 1) / (dl + dm));
+    const double b = sqrt((dl + dm - 1) / (dl - dm) * (2 * dl + 1) / (2 * dl - 3) * (dl - dm - 1) / (dl + dm));
+    const double v = x * a * v1 - b * v0;
+    v0 = v1;
+    v1 = v;
+    ++L;
+    if (e < 0 && fabs(v1) > 1.0) {
+      v0 *= 0x1p-600;
+      v1 *= 0x1p-600;
+      ++e;
+    }
+  }
+};
+
+// The two tables of the vector transform (sht_tools.py:267-304), one thread
+// per (m, ring pair k), in the layout of the scalar table: D0 at D, D1 at
+// D + vstride.
+//   D0[l] = 1/2 (sqrt((l+m)(l-m+1)) P^{m-1}_l - sqrt((l-m)(l+m+1)) P^{m+1}_l)
+//           (m = 0: -sqrt(l(l+1)) P^1_l), the theta-derivative of P^m_l;
+//   D1[l] = 1/2 sqrt((2l+1)/(2l+3)) (sqrt((l-m+1)(l-m+2)) P^{m-1}_{l+1}
+//           + sqrt((l+m+1)(l+m+2)) P^{m+1}_{l+1})  (m >= 1), = m P^m_l / sin;
+// both negated for odd m when csphase (the Condon-Shortley sign).  Only the
+// orders m - 1 and m + 1 are needed; each runs its own scaled recurrence.
+__global__ __launch_bounds__(kThreads) void vector_table_kernel(
+    float* __restrict__ D, int64_t vstride, const int64_t* __restrict__ p_off, const int* __restrict__ nl,
+    const double* __restrict__ ct, int npair, int kpad, int lmax, int csphase) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  const int m = blockIdx.y;
+  if (k >= kpad) return;
+  const int nl0 = nl[2 * m], nl1 = nl[2 * m + 1];
+  const int pad0 = (nl0 + kRowPad - 1) / kRowPad * kRowPad;
+  const int pad1 = (nl1 + kRowPad - 1) / kRowPad * kRowPad;
+  float* T0[2] = {D + p_off[m], D + p_off[m] + (int64_t)pad0 * kpad};
+  float* T1[2] = {T0[0] + vstride, T0[1] + vstride};
+  for (int i = nl0; i < pad0; ++i) T0[0][(int64_t)i * kpad + k] = T1[0][(int64_t)i * kpad + k] = 0.f;
+  for (int i = nl1; i < pad1; ++i) T0[1][(int64_t)i * kpad + k] = T1[1][(int64_t)i * kpad + k] = 0.f;
+  if (k >= npair) {
+    for (int i = 0; i < nl0; ++i) T0[0][(int64_t)i * kpad + k] = T1[0][(int64_t)i * kpad + k] = 0.f;
+    for (int i = 0; i < nl1; ++i) T0[1][(int64_t)i * kpad + k] = T1[1][(int64_t)i * kpad + k] = 0.f;
+    return;
+  }
+  if (m >= lmax) return;
+  const double x = ct[k];
+  const double s = sqrt((1.0 - x) * (1.0 + x));
+  const float sign = (csphase && (m & 1)) ? -1.f : 1.f;
+  const double dm = m;
+  ScaledOrder lo, hi;  // orders m - 1 and m + 1
+  hi.init(m + 1, x, s);
+  if (m > 0) {
+    lo.init(m - 1, x, s);
+    lo.advance(x);  // degrees m, m + 1
+  }
+  for (int l = m; l < lmax; ++l) {
+    const double dl = l;
+    double d0, d1;
+    if (m == 0) {
+      d0 = -sqrt(dl * (dl + 1)) * hi.at(l);
+      d1 = 0.0;
+    } else {
+      d0 = 0.5 * (sqrt((dl + dm) * (dl - dm + 1)) * lo.at(l) - sqrt((dl - dm) * (dl + dm + 1)) * hi.at(l));
+      d1 = 0.5 * sqrt((2 * dl + 1) / (2 * dl + 3)) *
+           (sqrt((dl - dm + 1) * (dl - dm + 2)) * lo.at(l + 1) + sqrt((dl + dm + 1) * (dl + dm + 2)) * hi.at(l + 1));
+      lo.advance(x);
+    }
+    if (l > m) hi.advance(x);  // keep degrees l + 1, l + 2 for the next l
+    const int par = (l - m) & 1;
+    const int64_t off = (int64_t)((l - m) >> 1) * kpad + k;
+    T0[par][off] = sign * (float)d0;
+    T1[par][off] = sign * (float)d1;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Forward: aT[m][p][i][col] = sum_k P[m][p][i][k] X[m][p][k][col]
 // ---------------------------------------------------------------------------
@@ -279,11 +385,26 @@ __device__ __forceinline__ void store_tiles_lds(const floatx4 (&acc)[2][4][CT], 
 // use allows without spilling.
 constexpr int gemm_min_waves(int prec, int ct) { return ct == 1 ? 4 : (prec == HPX_PREC_FP32 ? 3 : 2); }
 
-template <int PREC, int CT>
+// Lane signs and column swaps of the vector transform's GEMMs.  Columns are
+// grouped by field in fours, q = column & 3 = (component, re / im) with
+// component 0 = spheroidal / theta, 1 = toroidal / phi.  The reduction runs
+// over two halves: the D0 table against spectra (coefficients) of the
+// opposite parity block with column q and sign s0(q) = (+, +, -, -), then
+// the D1 table against the same parity block with column q ^ 3 and sign
+// s1(q) = (+, -, +, -) in the forward transform, (-, +, -, +) in the inverse
+// (hpx_sht.py:351-372 and 413-432 written out per real column).
+__device__ __forceinline__ float vec_sign0(int lane) { return (lane & 2) ? -1.f : 1.f; }
+__device__ __forceinline__ float vec_sign1(int lane, bool forward) {
+  return ((lane & 1) != 0) == forward ? -1.f : 1.f;
+}
+
+// VEC: the vector transform; the reduction is 2 kpad long (D0 half, then D1
+// half at P + vstride), see vec_sign0 / vec_sign1.
+template <int PREC, int CT, bool VEC = false>
 __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_fwd_kernel(
     const float* __restrict__ X, float* __restrict__ aT, const float* __restrict__ P,
     const int64_t* __restrict__ p_off, const int* __restrict__ nl,
-    const int2* __restrict__ items, int nitems, int ncb, int kpad, int64_t ncol) {
+    const int2* __restrict__ items, int nitems, int ncb, int kpad, int64_t ncol, int64_t vstride) {
   // table tiles (double buffer) during the main loop, the output tile after it
   constexpr size_t kTileBytes = sizeof(float) * 64 * (64 * CT + 4);
   constexpr size_t kLdsBytes = 2 * sizeof(FwdLds<PREC>) > kTileBytes ? 2 * sizeof(FwdLds<PREC>) : kTileBytes;
@@ -328,12 +449,16 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
   // pair are zero, so no k bound check
   const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + (int64_t)m * 2 * pstride, 2 * pstride * 4);
   const int bvoff = (frag_k_lane<PREC>(lane) * nc + col0 + (lane & 15)) * 4;
+  const int bvoff3 = (frag_k_lane<PREC>(lane) * nc + col0 + ((lane & 15) ^ 3)) * 4;
+  const float sgn0 = vec_sign0(lane), sgn1 = vec_sign1(lane, true);
 
   auto load_a = [&](int kc, float* v) {
+    const float* src = Psrc + kc;
+    if (VEC && kc >= kpad) src += vstride - kpad;
     if (srow_ok) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const float4 t = *reinterpret_cast<const float4*>(Psrc + kc + 4 * q);
+        const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
         v[4 * q] = t.x;
         v[4 * q + 1] = t.y;
         v[4 * q + 2] = t.z;
@@ -357,19 +482,33 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
     }
   };
   auto load_b = [&](int kc, float (&bv)[2][CT][8]) {
+    const bool half1 = VEC && kc >= kpad;
+    const int kk = half1 ? kc - kpad : kc;
+    const int voff = half1 ? bvoff3 : bvoff;
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int p = 0; p < 2; ++p) {
+      const int pb = VEC && !half1 ? 1 - p : p;  // spectra parity block
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
         for (int q = 0; q < 8; ++q)
-          bv[p][ct][q] = buffer_load(xr, bvoff,
-                                     (int)((p * pstride + (int64_t)(kc + frag_k_uniform<PREC>(q)) * nc + ct * 16) * 4));
+          bv[p][ct][q] = buffer_load(xr, voff,
+                                     (int)((pb * pstride + (int64_t)(kk + frag_k_uniform<PREC>(q)) * nc + ct * 16) * 4));
+    }
+    if constexpr (VEC) {
+      const float sg = half1 ? sgn1 : sgn0;
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) bv[p][ct][q] *= sg;
+    }
   };
 
   // software pipeline over k chunks with two register sets for B and two
   // LDS buffers for the table tile (one barrier per chunk, no copies)
-  const int nchunk = kpad / kKTile;
+  const int nchunk = (VEC ? 2 : 1) * kpad / kKTile;
   float pa[16];
   float b0[2][CT][8], b1[2][CT][8];
   load_a(0, pa);
@@ -411,11 +550,14 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
 // row half + column wave).  With RH = 2 a work-group covers 128 ring pairs,
 // so each coefficient chunk serves both halves and the per-work-group set-up
 // and output staging are spread over twice the work.
-template <int PREC, int CT, int RH>
+// VEC: the vector transform; the reduction runs over the rows of both
+// parities (D0T half against the opposite parity's coefficients, then the
+// D1T half at PT + vstride against the same parity's), see vec_sign0.
+template <int PREC, int CT, int RH, bool VEC = false>
 __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_kernel(
     const float* __restrict__ aT, float* __restrict__ X, const float* __restrict__ PT,
     const int64_t* __restrict__ p_off, const int* __restrict__ nl, int nkb, int nitems, int ncb, int kpad,
-    int64_t ncol, int64_t at_rows) {
+    int64_t ncol, int64_t at_rows, int64_t vstride) {
   // same tiling as the forward kernel with the roles of i and k exchanged:
   // rows = ring pairs, reduction over the rows i of aT, A tiles from the
   // transposed table PT[m][p][k][i]
@@ -461,16 +603,26 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
   const int sp = threadIdx.x / (2 * kRows);
   const int sr = (threadIdx.x >> 1) & (kRows - 1);
   const int sh = (threadIdx.x & 1) * 16;
-  const int spad = padp[sp];
-  const float* Pbase = PT + p_off[m] + (sp ? (int64_t)padp[0] * kpad : 0) + (int64_t)sr * spad + sh;
+  // first half: parity sp (scalar) or 1 - sp (vector, D0T); second half
+  // (vector only): parity sp of D1T
+  const int sp0 = VEC ? 1 - sp : sp;
+  const int spad0 = padp[sp0], spad1 = padp[sp];
+  const float* Pbase0 = PT + p_off[m] + (sp0 ? (int64_t)padp[0] * kpad : 0) + (int64_t)sr * spad0 + sh;
+  const float* Pbase1 = PT + vstride + p_off[m] + (sp ? (int64_t)padp[0] * kpad : 0) + (int64_t)sr * spad1 + sh;
   const int64_t row0m = p_off[m] / kpad;
   const __amdgpu_buffer_rsrc_t ar = buffer_rsrc(aT + row0m * ncol, (at_rows + kAtSlackRows - row0m) * ncol * 4);
   const int bvoff = (frag_k_lane<PREC>(lane) * nc + col0 + (lane & 15)) * 4;
+  const int bvoff3 = (frag_k_lane<PREC>(lane) * nc + col0 + ((lane & 15) ^ 3)) * 4;
+  const float sgn0 = vec_sign0(lane), sgn1 = vec_sign1(lane, false);
+  const int kp = (nmax + kKTile - 1) / kKTile * kKTile;  // length of one half
 
   // table tile: this block's 64 ring pairs, i chunk starting at ic
   auto load_a = [&](int ic, float* v) {
-    if (k0 + sr < kpad && ic + sh < spad) {
-      const float* src = Pbase + (int64_t)k0 * spad + ic;
+    const bool half1 = VEC && ic >= kp;
+    const int ii = half1 ? ic - kp : ic;
+    const int spad = half1 ? spad1 : spad0;
+    if (k0 + sr < kpad && ii + sh < spad) {
+      const float* src = (half1 ? Pbase1 : Pbase0) + (int64_t)k0 * spad + ii;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
@@ -501,17 +653,31 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
   // rows, or the zero slack after aT: finite values that meet zero table
   // entries (PT is zero for i >= the padded count), so no bound check.
   auto load_b = [&](int ic, float (&bv)[2][CT][8]) {
+    const bool half1 = VEC && ic >= kp;
+    const int ii = half1 ? ic - kp : ic;
+    const int voff = half1 ? bvoff3 : bvoff;
 #pragma unroll
-    for (int p = 0; p < 2; ++p)
+    for (int p = 0; p < 2; ++p) {
+      const int pb = VEC && !half1 ? 1 - p : p;  // coefficient parity block
 #pragma unroll
       for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
         for (int q = 0; q < 8; ++q)
           bv[p][ct][q] = buffer_load(
-              ar, bvoff, (int)(((int64_t)((p ? padp[0] : 0) + ic + frag_k_uniform<PREC>(q)) * nc + ct * 16) * 4));
+              ar, voff, (int)(((int64_t)((pb ? padp[0] : 0) + ii + frag_k_uniform<PREC>(q)) * nc + ct * 16) * 4));
+    }
+    if constexpr (VEC) {
+      const float sg = half1 ? sgn1 : sgn0;
+#pragma unroll
+      for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) bv[p][ct][q] *= sg;
+    }
   };
 
-  const int nchunk = (nmax + kKTile - 1) / kKTile;  // 0 for m >= lmax: zero spectra
+  const int nchunk = (VEC ? 2 : 1) * kp / kKTile;  // 0 for m >= lmax: zero spectra
   if (nchunk > 0) {
     float pa[16];
     float b0[2][CT][8], b1[2][CT][8];
@@ -594,9 +760,11 @@ __device__ __forceinline__ int64_t at_row(const int64_t* p_off, const int* nl, i
 }
 
 // a[b][l][m] from aT; zero for l < m < mmax (the reference's dense layout).
+// lnorm: scale by 1 / (l (l + 1)) (1 at l = 0), the vector transform's
+// normalisation (hpx_sht.py:323-328).
 __global__ __launch_bounds__(kThreads) void coeff_to_user_kernel(
     const float* __restrict__ aT, float* __restrict__ alm, const int64_t* __restrict__ p_off,
-    const int* __restrict__ nl, int kpad, int lmax, int mmax, int64_t ncol, int64_t nbatch) {
+    const int* __restrict__ nl, int kpad, int lmax, int mmax, int64_t ncol, int64_t nbatch, int lnorm) {
   __shared__ float t[kTrM][kTrCols + 1];
   const int l = blockIdx.x;
   const int m0 = blockIdx.y * kTrM;
@@ -627,12 +795,13 @@ __global__ __launch_bounds__(kThreads) void coeff_to_user_kernel(
   const int m = m0 + mm;
   if (m >= mmax) return;
   const bool zero = m >= mend;
+  const float sc = (lnorm && l > 0) ? (float)(1.0 / ((double)l * (l + 1))) : 1.f;
 #pragma unroll
   for (int j = 0; j < 8; ++j) {
     const int bs = (tid >> 5) + 8 * j;  // field within the tile
     const int64_t b = c0 / 2 + bs;
     if (b >= nbatch) break;
-    const float2 v = zero ? make_float2(0.f, 0.f) : make_float2(t[mm][2 * bs], t[mm][2 * bs + 1]);
+    const float2 v = zero ? make_float2(0.f, 0.f) : make_float2(sc * t[mm][2 * bs], sc * t[mm][2 * bs + 1]);
     *reinterpret_cast<float2*>(alm + 2 * ((b * lmax + l) * (int64_t)mmax + m)) = v;
   }
 }
@@ -707,8 +876,13 @@ __global__ __launch_bounds__(kThreads) void user_to_coeff_kernel(
 
 int launch_legendre_table(ShtPlan& p, const double* d_ct, hipStream_t s) {
   dim3 grid((p.kpad + kThreads - 1) / kThreads, p.mmax);
-  legendre_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
-                                                  p.lmax);
+  if (p.vector) {
+    vector_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.vstride, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
+                                                  p.lmax, p.csphase);
+  } else {
+    legendre_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
+                                                    p.lmax);
+  }
   HPX_LAUNCH_CHECK();
   if (p.d_pt != nullptr && p.mmax > 0) {
     int max_pad = 0;
@@ -718,8 +892,11 @@ int launch_legendre_table(ShtPlan& p, const double* d_ct, hipStream_t s) {
     if (max_pad > 0) {
       if (2 * p.mmax > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "mmax too large");
       const dim3 tg((unsigned)(p.kpad / 32), (unsigned)((max_pad + 31) / 32), (unsigned)(2 * p.mmax));
-      table_transpose_kernel<<<tg, kThreads, 0, s>>>(p.d_p, p.d_pt, p.d_p_off, p.d_nl, p.kpad);
-      HPX_LAUNCH_CHECK();
+      for (int t = 0; t < (p.vector ? 2 : 1); ++t) {
+        table_transpose_kernel<<<tg, kThreads, 0, s>>>(p.d_p + t * p.vstride, p.d_pt + t * p.vstride, p.d_p_off,
+                                                       p.d_nl, p.kpad);
+        HPX_LAUNCH_CHECK();
+      }
     }
   }
   return HPX_OK;
@@ -737,9 +914,15 @@ int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, flo
   const int64_t ncb = (ncol + 64 * ct - 1) / (64 * ct);
   if ((int64_t)p.n_fwd_items * ncb > 0x7ffffff0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   const dim3 grid(xcd_grid((int64_t)p.n_fwd_items * ncb));
-#define HPX_FWD(PR, CT)                                                                  \
-  legendre_fwd_kernel<PR, CT><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, \
-                                                        p.d_fwd_items, p.n_fwd_items, (int)ncb, p.kpad, ncol)
+#define HPX_FWD(PR, CT)                                                                                       \
+  do {                                                                                                        \
+    if (p.vector)                                                                                             \
+      legendre_fwd_kernel<PR, CT, true><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items, \
+                                                                  p.n_fwd_items, (int)ncb, p.kpad, ncol, p.vstride); \
+    else                                                                                                      \
+      legendre_fwd_kernel<PR, CT><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items,  \
+                                                            p.n_fwd_items, (int)ncb, p.kpad, ncol, 0);       \
+  } while (0)
   switch (precision * 4 + ct) {
     case HPX_PREC_FP32 * 4 + 1: HPX_FWD(HPX_PREC_FP32, 1); break;
     case HPX_PREC_FP32 * 4 + 2: HPX_FWD(HPX_PREC_FP32, 2); break;
@@ -758,15 +941,23 @@ int launch_gemm_inv(const ShtPlan& p, int precision, int ct, int rh, const float
                     int64_t nbatch, hipStream_t s) {
   if (nbatch <= 0) return HPX_OK;
   if (rh != 1 && rh != 2) HPX_FAIL(HPX_ERR_ARG, "no inverse GEMM with %d row halves", rh);
+  if (p.vector) rh = 1;  // the vector variants are built for 64 ring pairs per work-group
   const int64_t ncol = col_pad(nbatch);
   const int64_t ncb = (ncol + 64 * ct - 1) / (64 * ct);
   const int nkb = (p.kpad + 64 * rh - 1) / (64 * rh);
   const int nitems = nkb * p.mmax;
   if ((int64_t)nitems * ncb > 0x7ffffff0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   const dim3 grid(xcd_grid((int64_t)nitems * ncb));
-#define HPX_INV(PR, CT, RH)                                                                                  \
-  legendre_inv_kernel<PR, CT, RH><<<grid, kThreads * RH, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, nkb, nitems, \
-                                                                 (int)ncb, p.kpad, ncol, p.at_rows)
+#define HPX_INV(PR, CT, RH)                                                                                    \
+  do {                                                                                                         \
+    if (RH == 1 && p.vector)                                                                                   \
+      legendre_inv_kernel<PR, CT, 1, true><<<grid, kThreads, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, nkb,    \
+                                                                     nitems, (int)ncb, p.kpad, ncol, p.at_rows, \
+                                                                     p.vstride);                                \
+    else                                                                                                       \
+      legendre_inv_kernel<PR, CT, RH><<<grid, kThreads * RH, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, nkb,     \
+                                                                     nitems, (int)ncb, p.kpad, ncol, p.at_rows, 0); \
+  } while (0)
   switch (precision * 8 + ct * 2 + rh - 1) {
     case HPX_PREC_FP32 * 8 + 2: HPX_INV(HPX_PREC_FP32, 1, 1); break;
     case HPX_PREC_FP32 * 8 + 3: HPX_INV(HPX_PREC_FP32, 1, 2); break;
@@ -793,7 +984,7 @@ int launch_coeff_to_user(const ShtPlan& p, const float* aT, float* alm, int64_t 
   const int64_t ncol = col_pad(nbatch);
   if ((ncol + kTrCols - 1) / kTrCols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   coeff_to_user_kernel<<<transpose_grid(p, ncol), kThreads, 0, s>>>(aT, alm, p.d_p_off, p.d_nl, p.kpad,
-                                                                   p.lmax, p.mmax, ncol, nbatch);
+                                                                   p.lmax, p.mmax, ncol, nbatch, p.vector);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }

@@ -99,8 +99,10 @@ void free_plan(ShtPlan* p) {
 
 // Builds a plan from ring geometry; cos_theta[t] is used for the Legendre
 // table of the north rings (t < npair).
+// vector: build the two tables of the vector transform (D0, D1) instead of P.
 int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const double* phi0,
-               const double* weights, int lmax, int mmax, hipStream_t stream, ShtPlan** out) {
+               const double* weights, int lmax, int mmax, hipStream_t stream, ShtPlan** out,
+               int vector = 0, int csphase = 1) {
   if (nring < 1) HPX_FAIL(HPX_ERR_ARG, "nring must be >= 1");
   if (lmax < 0 || mmax < 1) HPX_FAIL(HPX_ERR_ARG, "lmax must be >= 0 and mmax >= 1 (got %d, %d)", lmax, mmax);
   if (mmax > 65535 || lmax > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "lmax/mmax too large");
@@ -120,6 +122,8 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   p->kpad = (p->npair + kKTile - 1) / kKTile * kKTile;
   p->lmax = lmax;
   p->mmax = mmax;
+  p->vector = vector ? 1 : 0;
+  p->csphase = csphase ? 1 : 0;
 
   std::vector<int64_t> offset(nring + 1, 0);
   for (int t = 0; t < nring; ++t) offset[t + 1] = offset[t] + nphi[t];
@@ -249,10 +253,11 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   }
   const int64_t pfloats = p->p_off[mmax];
   p->at_rows = pfloats / p->kpad;
+  p->vstride = p->vector ? pfloats : 0;
+  const int64_t tfloats = std::max<int64_t>((p->vector ? 2 : 1) * pfloats, 1);
   double* d_ct = nullptr;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_p), std::max<int64_t>(pfloats, 1) * sizeof(float));
-  if (e == hipSuccess)
-    e = hipMalloc(reinterpret_cast<void**>(&p->d_pt), std::max<int64_t>(pfloats, 1) * sizeof(float));
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_p), tfloats * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->d_pt), tfloats * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_ct), ct.size() * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(d_ct, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -260,7 +265,7 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     free_plan(p);
     HPX_FAIL(HPX_ERR_HIP, "plan allocation failed: %s", hipGetErrorString(e));
   }
-  bytes += 2 * pfloats * (int64_t)sizeof(float);
+  bytes += 2 * tfloats * (int64_t)sizeof(float);
   rc = launch_legendre_table(*p, d_ct, stream);
   if (rc == HPX_OK) {
     e = hipStreamSynchronize(stream);
@@ -279,9 +284,12 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   return HPX_OK;
 }
 
-int check_call(const ShtPlan* p, int precision, int64_t nbatch, int64_t ws_bytes) {
+// nbatch: scalar fields (2 per vector field for the vector calls)
+int check_call(const ShtPlan* p, int precision, int64_t nbatch, int64_t ws_bytes, bool vector_call = false) {
   if (p == nullptr) HPX_FAIL(HPX_ERR_ARG, "null plan");
   if (p->lmax < 1) HPX_FAIL(HPX_ERR_ARG, "plan was built with lmax = 0 (ring FFTs only)");
+  if (p->vector && !vector_call) HPX_FAIL(HPX_ERR_ARG, "plan holds vector-transform tables (use hpx_vsht_*)");
+  if (!p->vector && vector_call) HPX_FAIL(HPX_ERR_ARG, "plan holds the scalar table (use hpx_vsht_plan_create_healpix)");
   if (precision < 0 || precision > 2) HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
   if (nbatch < 0) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 0");
   const int64_t need = workspace_floats(*p, nbatch) * (int64_t)sizeof(float);
@@ -319,8 +327,9 @@ extern "C" int hpx_sht_plan_create_rings(int nring, const int32_t* nphi, const d
   return HPX_OK;
 }
 
-extern "C" int hpx_sht_plan_create_healpix(int nside, int lmax, int mmax, const double* ring_weights,
-                                           hipStream_t stream, hpx_sht_plan** plan) {
+namespace {
+int healpix_plan(int nside, int lmax, int mmax, const double* ring_weights, hipStream_t stream,
+                 hpx_sht_plan** plan, int vector, int csphase) {
   if (plan == nullptr || ring_weights == nullptr) HPX_FAIL(HPX_ERR_ARG, "null argument");
   if (nside < 1) HPX_FAIL(HPX_ERR_ARG, "nside must be >= 1");
   if (nside > 2048) HPX_FAIL(HPX_ERR_UNSUPPORTED, "nside > 2048 exceeds the FFT size limit");
@@ -348,10 +357,54 @@ extern "C" int hpx_sht_plan_create_healpix(int nside, int lmax, int mmax, const 
       phi0[t] = M_PI / (2.0 * (t + 1)) * 0.5;
   }
   ShtPlan* p = nullptr;
-  const int rc = build_plan(nring, nphi.data(), z.data(), phi0.data(), ring_weights, lmax, mmax, stream, &p);
+  const int rc = build_plan(nring, nphi.data(), z.data(), phi0.data(), ring_weights, lmax, mmax, stream, &p,
+                            vector, csphase);
   if (rc != HPX_OK) return rc;
   *plan = H(p);
   return HPX_OK;
+}
+}  // namespace
+
+extern "C" int hpx_sht_plan_create_healpix(int nside, int lmax, int mmax, const double* ring_weights,
+                                           hipStream_t stream, hpx_sht_plan** plan) {
+  return healpix_plan(nside, lmax, mmax, ring_weights, stream, plan, 0, 1);
+}
+
+extern "C" int hpx_vsht_plan_create_healpix(int nside, int lmax, int mmax, const double* ring_weights,
+                                            int csphase, hipStream_t stream, hpx_sht_plan** plan) {
+  if (lmax < 1) HPX_FAIL(HPX_ERR_ARG, "a vector plan needs lmax >= 1");
+  return healpix_plan(nside, lmax, mmax, ring_weights, stream, plan, 1, csphase);
+}
+
+extern "C" int64_t hpx_vsht_workspace_bytes(const hpx_sht_plan* plan, int64_t nbatch) {
+  if (plan == nullptr || nbatch < 0 || nbatch > (INT64_MAX >> 2)) return -1;
+  return workspace_floats(*P(plan), 2 * nbatch) * (int64_t)sizeof(float);
+}
+
+extern "C" int hpx_vsht_forward(const hpx_sht_plan* plan, int precision, const float* f, float* alm,
+                                int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                                hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  if (nbatch < 0 || nbatch > (INT64_MAX >> 2)) HPX_FAIL(HPX_ERR_ARG, "bad nbatch");
+  int rc = check_call(pl, precision, 2 * nbatch, workspace_bytes, true);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  // the two components of each field are consecutive maps: 2 nbatch scalar
+  // ring analyses (quadrature weight applied), then the vector GEMM
+  float* X = static_cast<float*>(workspace);
+  if ((rc = launch_analysis(*pl, f, X, 2 * nbatch, 1, 1.0f, 0, stream)) != HPX_OK) return rc;
+  return launch_legendre_fwd(*pl, precision, X, alm, 2 * nbatch, stream);
+}
+
+extern "C" int hpx_vsht_inverse(const hpx_sht_plan* plan, int precision, const float* alm, float* f,
+                                int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                                hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  if (nbatch < 0 || nbatch > (INT64_MAX >> 2)) HPX_FAIL(HPX_ERR_ARG, "bad nbatch");
+  int rc = check_call(pl, precision, 2 * nbatch, workspace_bytes, true);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  float* X = static_cast<float*>(workspace);
+  if ((rc = launch_legendre_inv(*pl, precision, alm, X, 2 * nbatch, stream)) != HPX_OK) return rc;
+  return launch_synthesis(*pl, X, f, 2 * nbatch, 0, 1.0f, 0, stream);
 }
 
 extern "C" void hpx_sht_plan_destroy(hpx_sht_plan* plan) { free_plan(P(plan)); }

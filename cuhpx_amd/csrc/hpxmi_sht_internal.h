@@ -75,6 +75,12 @@ struct ShtPlan {
   int n_fwd_items = 0;
   int64_t at_rows = 0;  // rows of the P table = rows of the coefficient workspace aT
   int64_t table_bytes = 0;
+  // Vector plans (VectorSHT / VectoriSHT) hold two tables in the layout of P,
+  // D0 (theta-derivative) at d_p and D1 (m P / sin theta) at d_p + vstride
+  // (the same for d_pt), instead of P.
+  int vector = 0;
+  int csphase = 1;
+  int64_t vstride = 0;
 
   // Autotuned launch shapes (hpx_sht_autotune); untuned keys use defaults.
   mutable std::mutex tune_mu;

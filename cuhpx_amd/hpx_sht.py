@@ -40,15 +40,17 @@ _plans = {}
 
 
 class ShtPlan:
-    """A device plan of libhpxmi.so for one grid, lmax, mmax and device."""
+    """A device plan of libhpxmi.so for one grid, lmax, mmax and device
+    (scalar, or with ``vector`` the tables of the vector transform)."""
 
-    def __init__(self, handle, device, npix, nring, lmax, mmax):
+    def __init__(self, handle, device, npix, nring, lmax, mmax, vector=False):
         self._handle = handle
         self.device = device
         self.npix = npix
         self.nring = nring
         self.lmax = lmax
         self.mmax = mmax
+        self.vector = vector
 
     @property
     def handle(self):
@@ -70,6 +72,26 @@ class ShtPlan:
                         _lib.stream_ptr(device), ctypes.byref(h))
                 _lib.check(rc, "SHT plan")
                 plan = cls(h, device, 12 * nside * nside, 4 * nside - 1, lmax, mmax)
+                _plans[key] = plan
+        return plan
+
+    @classmethod
+    def healpix_vector(cls, nside, lmax, mmax, quad_weights="ring", csphase=True, device=None):
+        """Plan of the vector transform (VectorSHT / VectoriSHT tables)."""
+        device = torch.device("cuda", torch.cuda.current_device() if device is None else device.index)
+        key = ("healpix-vector", nside, lmax, mmax, quad_weights, bool(csphase), device.index)
+        with _plan_lock:
+            plan = _plans.get(key)
+            if plan is None:
+                _, w = healpix_weights(nside, quad_weights)
+                w = np.ascontiguousarray(w, dtype=np.float64)
+                h = ctypes.c_void_p()
+                with torch.cuda.device(device):
+                    rc = _lib.lib().hpx_vsht_plan_create_healpix(
+                        nside, lmax, mmax, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(bool(csphase)),
+                        _lib.stream_ptr(device), ctypes.byref(h))
+                _lib.check(rc, "vector SHT plan")
+                plan = cls(h, device, 12 * nside * nside, 4 * nside - 1, lmax, mmax, vector=True)
                 _plans[key] = plan
         return plan
 
@@ -112,7 +134,9 @@ class ShtPlan:
         return int(_lib.lib().hpx_sht_plan_table_bytes(self._handle))
 
     def workspace(self, nbatch):
-        nbytes = int(_lib.lib().hpx_sht_workspace_bytes(self._handle, nbatch))
+        """Workspace for nbatch maps (scalar plan) or nbatch vector fields."""
+        fn = _lib.lib().hpx_vsht_workspace_bytes if self.vector else _lib.lib().hpx_sht_workspace_bytes
+        nbytes = int(fn(self._handle, nbatch))
         return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device), nbytes
 
     def autotune(self, precision, nbatch):
@@ -156,6 +180,24 @@ class ShtPlan:
         rc = self._run(_lib.lib().hpx_sht_inverse, _lib.PRECISIONS[precision], _lib.ptr(alm),
                        _lib.ptr(out), nbatch=n)
         _lib.check(rc, "SHT inverse")
+        return out
+
+    def vector_forward(self, f, precision):
+        """f [n, 2, npix] float32 contiguous -> alm [n, 2, lmax, mmax] complex64."""
+        n = f.shape[0]
+        out = torch.empty((n, 2, self.lmax, self.mmax), dtype=torch.complex64, device=f.device)
+        rc = self._run(_lib.lib().hpx_vsht_forward, _lib.PRECISIONS[precision], _lib.ptr(f),
+                       _lib.ptr(out), nbatch=n)
+        _lib.check(rc, "vector SHT forward")
+        return out
+
+    def vector_inverse(self, alm, precision):
+        """alm [n, 2, lmax, mmax] complex64 contiguous -> f [n, 2, npix] float32."""
+        n = alm.shape[0]
+        out = torch.empty((n, 2, self.npix), dtype=torch.float32, device=alm.device)
+        rc = self._run(_lib.lib().hpx_vsht_inverse, _lib.PRECISIONS[precision], _lib.ptr(alm),
+                       _lib.ptr(out), nbatch=n)
+        _lib.check(rc, "vector SHT inverse")
         return out
 
     def forward_adjoint(self, galm, precision, mode):
@@ -287,3 +329,89 @@ class iSHTCUDA(_HealpixSHTBase):
         if tuple(x.shape[-2:]) != (plan.lmax, plan.mmax):
             raise RuntimeError(f"last dimensions must be ({plan.lmax}, {plan.mmax}), got {tuple(x.shape[-2:])}")
         return iSHTFunction.apply(x, plan, self.precision, self.adjoint)
+
+
+class _HealpixVectorBase(nn.Module):
+    def __init__(self, nside, lmax, mmax, grid, quad_weights, norm, csphase, use_bluestein, precision):
+        super().__init__()
+        if grid != "healpix":
+            raise ValueError("Unknown quadrature mode")
+        if precision not in _lib.PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(_lib.PRECISIONS)}")
+        self.nside = nside
+        self.grid = grid
+        self.norm = norm
+        self.csphase = csphase
+        self.nlat = 4 * nside - 1
+        self.nlon = 4 * nside
+        self.quad_weights = quad_weights
+        self.use_bluestein = use_bluestein
+        self.lmax = lmax or self.nlat
+        self.mmax = mmax or (self.nlon // 2 + 1)
+        self.precision = precision
+        if not torch.cuda.is_available():
+            raise RuntimeError("CUDA device is not available. This class requires a GPU.")
+        self._plan = ShtPlan.healpix_vector(nside, self.lmax, self.mmax, quad_weights, csphase)
+
+    def plan_for(self, device):
+        if device.type != "cuda":
+            raise RuntimeError("VectorSHT / VectoriSHT need a GPU tensor")
+        if self._plan.device.index != device.index:
+            self._plan = ShtPlan.healpix_vector(self.nside, self.lmax, self.mmax, self.quad_weights, self.csphase,
+                                                device)
+        return self._plan
+
+
+class VectorSHT(_HealpixVectorBase):
+    """Forward vector SHT (cuhpx/hpx_sht.py:286-374): real [..., 2, 12 nside^2]
+    (theta and phi components, RING) -> complex [..., 2, lmax, mmax]
+    (spheroidal, toroidal coefficients).
+
+    Same constructor as the reference (``grid`` must be "healpix";
+    ``norm`` and ``use_bluestein`` are accepted and, as there, have no
+    effect for the orthonormal HEALPix transform; ``csphase`` applies the
+    Condon-Shortley sign to odd m, default True as in sht_tools.py:302-304),
+    plus ``precision`` ("fp32", "bf16x3", "bf16") for the Legendre step.
+    Like the reference, whose ring FFTs carry no autograd, the output does not
+    track gradients."""
+
+    def __init__(self, nside, lmax=None, mmax=None, grid="healpix", quad_weights="ring", norm="ortho",
+                 csphase=True, use_bluestein=False, precision="fp32"):
+        super().__init__(nside, lmax, mmax, grid, quad_weights, norm, csphase, use_bluestein, precision)
+
+    def forward(self, x):
+        if torch.is_complex(x):
+            raise ValueError("Input tensor must be real.")
+        if x.dtype != torch.float32:
+            raise TypeError(f"VectorSHT computes in float32; got {x.dtype}")
+        plan = self.plan_for(x.device)
+        if x.dim() < 2 or x.shape[-2] != 2 or x.shape[-1] != plan.npix:
+            raise RuntimeError(f"input must be [..., 2, {plan.npix}], got {tuple(x.shape)}")
+        lead = x.shape[:-2]
+        with torch.no_grad():
+            out = plan.vector_forward(x.detach().reshape(-1, 2, plan.npix).contiguous(), self.precision)
+        return out.reshape(*lead, 2, plan.lmax, plan.mmax)
+
+
+class VectoriSHT(_HealpixVectorBase):
+    """Inverse vector SHT (cuhpx/hpx_sht.py:377-444): complex
+    [..., 2, lmax, mmax] (spheroidal, toroidal) -> real [..., 2, 12 nside^2]
+    (theta and phi components, RING).  Constructor as the reference's plus
+    ``precision``; the output does not track gradients."""
+
+    def __init__(self, nside, lmax=None, mmax=None, grid="healpix", norm="ortho", csphase=True,
+                 use_bluestein=False, precision="fp32"):
+        super().__init__(nside, lmax, mmax, grid, "ring", norm, csphase, use_bluestein, precision)
+
+    def forward(self, x):
+        if not torch.is_complex(x):
+            raise ValueError("Input tensor must be complex.")
+        if x.dtype != torch.complex64:
+            raise TypeError(f"VectoriSHT computes in complex64; got {x.dtype}")
+        plan = self.plan_for(x.device)
+        if x.dim() < 3 or tuple(x.shape[-3:]) != (2, plan.lmax, plan.mmax):
+            raise RuntimeError(f"input must be [..., 2, {plan.lmax}, {plan.mmax}], got {tuple(x.shape)}")
+        lead = x.shape[:-3]
+        with torch.no_grad():
+            out = plan.vector_inverse(x.detach().reshape(-1, 2, plan.lmax, plan.mmax).contiguous(), self.precision)
+        return out.reshape(*lead, 2, plan.npix)

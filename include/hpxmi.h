@@ -164,6 +164,42 @@ int hpx_sht_inverse_adjoint(const hpx_sht_plan* plan, int precision, int mode,
                             void* workspace, int64_t workspace_bytes,
                             hipStream_t stream);
 
+/* ------------------------------------------------------------------ */
+/* Vector (spheroidal / toroidal) SHT                                  */
+/* ------------------------------------------------------------------ */
+
+/* Plan for the vector transform on a HEALPix grid: the state VectorSHT /
+ * VectoriSHT build in their constructors (cuhpx/hpx_sht.py:288-330, 379-402):
+ * ring geometry and weights as hpx_sht_plan_create_healpix, and instead of
+ * the scalar table the two tables of _precompute_dlegpoly
+ * (cuhpx/sht_tools.py:267-304), D0 = dP/dtheta and D1 = m P / sin(theta),
+ * generated on the device in FP64 (Condon-Shortley sign on odd m when
+ * csphase != 0, the reference's default).  The scalar entry points reject a
+ * vector plan and the vector ones a scalar plan.  Synchronises the stream. */
+int hpx_vsht_plan_create_healpix(int nside, int lmax, int mmax,
+                                 const double* ring_weights, int csphase,
+                                 hipStream_t stream, hpx_sht_plan** plan);
+
+/* Workspace (bytes) for nbatch vector fields (that of 2*nbatch scalar maps). */
+int64_t hpx_vsht_workspace_bytes(const hpx_sht_plan* plan, int64_t nbatch);
+
+/* Forward vector SHT, f[nbatch, 2, npix] (theta and phi components) ->
+ * alm[nbatch, 2, lmax, mmax] (complex64; spheroidal, toroidal).
+ * Replaces VectorSHT.forward (hpx_sht.py:333-374): cuhpx_fft
+ * healpix_rfft_batch of both components, then the four real einsums per
+ * output part with weights dpct * w / (l (l + 1)). */
+int hpx_vsht_forward(const hpx_sht_plan* plan, int precision, const float* f,
+                     float* alm, int64_t nbatch, void* workspace,
+                     int64_t workspace_bytes, hipStream_t stream);
+
+/* Inverse vector SHT, alm[nbatch, 2, lmax, mmax] -> f[nbatch, 2, npix].
+ * Replaces VectoriSHT.forward (hpx_sht.py:404-444): the four real einsums
+ * per component, then cuhpx_fft healpix_irfft_batch.  The input is not
+ * modified. */
+int hpx_vsht_inverse(const hpx_sht_plan* plan, int precision, const float* alm,
+                     float* f, int64_t nbatch, void* workspace,
+                     int64_t workspace_bytes, hipStream_t stream);
+
 /* Per-ring real FFTs alone, in the reference's ring-major layout
  * ftm[nbatch, nring, mmax] (complex64), where mmax is the plan's mmax
  * (the reference's L).  A plan built with lmax = 0 has no Legendre table and
@@ -210,7 +246,8 @@ int hpx_sht_stage_ring_synthesis(const hpx_sht_plan* plan, float* f, int64_t nba
  * each FFT size class, both directions) and of the two Legendre GEMMs
  * (column tiles per wavefront) for this plan, precision and batch on the
  * stream's device, and keeps the fastest in the plan; later calls with the
- * same precision and batch column count use them.  Synchronises the stream;
+ * same precision and batch column count use them.  On a vector plan nbatch
+ * counts scalar maps (2 per vector field).  Synchronises the stream;
  * must not be called while the stream is being captured.  No counterpart in
  * the reference (its launch shapes are fixed, hpx_fft_cuda.cu:1062-1332). */
 int hpx_sht_autotune(hpx_sht_plan* plan, int precision, int64_t nbatch, void* workspace,
