@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--regrid", action="store_true",
                     help="measure differentiable regridding (nside 256 <-> 512x1024 lat/lon, 16 fields, "
                          "MSE loss, forward + backward) instead")
+    ap.add_argument("--vector", action="store_true",
+                    help="measure the vector SHT (VectorSHT + VectoriSHT) at nside 256, lmax = mmax = 513, "
+                         "32 vector fields instead")
     ap.add_argument("--mixed", action="store_true",
                     help="measure the mixed-size request (32 fields each at nside 64, 128, 256, 512, "
                          "lmax = mmax = 2 nside + 1) through one HIP graph launch instead")
@@ -337,6 +340,77 @@ def bench_regrid(args):
         "grad_rel_l2_vs_fp64": acc}))
 
 
+VEC_FIELDS = 32
+
+
+def bench_vector(args):
+    """Vector SHT: 32 band-limited (theta, phi) fields at nside 256, lmax =
+    mmax = 513; one step = forward (maps -> spheroidal / toroidal
+    coefficients) + inverse, through the C ABI on preallocated buffers."""
+    from cuhpx_amd import _lib
+    from cuhpx_amd.hpx_sht import ShtPlan
+    lib = _lib.lib()
+    t0 = time.perf_counter()
+    plan = ShtPlan.healpix_vector(NSIDE, LMAX, MMAX)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    g = torch.Generator().manual_seed(7)
+    lcol = torch.arange(LMAX, dtype=torch.float64)[:, None]
+    amp = torch.sqrt((lcol + 1.0) ** -2 / 2)
+    c0 = torch.complex(torch.randn(VEC_FIELDS, 2, LMAX, MMAX, generator=g, dtype=torch.float64),
+                       torch.randn(VEC_FIELDS, 2, LMAX, MMAX, generator=g, dtype=torch.float64)) * amp
+    c0[..., 0] = c0[..., 0].real.to(torch.complex128)
+    c0[..., torch.arange(LMAX)[:, None] < torch.arange(MMAX)[None, :]] = 0
+    c0[:, :, 0, :] = 0
+    c0 = c0.to(torch.complex64).cuda()
+    f = plan.vector_inverse(c0, "fp32")  # band-limited synthetic vector fields in HBM
+    alm = torch.empty_like(c0)
+    out = torch.empty_like(f)
+    res = {}
+    for prec in ("bf16x3", "fp32"):
+        shapes = plan.autotune(prec, 2 * VEC_FIELDS)
+        ws, nbytes = plan.workspace(VEC_FIELDS)
+        stream = _lib.stream_ptr()
+        pr = _lib.PRECISIONS[prec]
+        wsp = _lib.ptr(ws)
+
+        def step():
+            _lib.check(lib.hpx_vsht_forward(plan.handle, pr, _lib.ptr(f), _lib.ptr(alm), VEC_FIELDS, wsp, nbytes,
+                                            stream), "vector forward")
+            _lib.check(lib.hpx_vsht_inverse(plan.handle, pr, _lib.ptr(alm), _lib.ptr(out), VEC_FIELDS, wsp, nbytes,
+                                            stream), "vector inverse")
+
+        ms = time_events(step, args.steps, args.warmup)
+        res[prec] = {"ms_per_step": ms, "fields_per_s": VEC_FIELDS / (ms * 1e-3), "tuned_shapes": shapes,
+                     "alm": alm[:1].cpu().numpy(), "inv": plan.vector_inverse(c0[:1].contiguous(), prec).cpu().numpy()}
+        print(f"[vector {prec}] {ms:.3f} ms per step", file=sys.stderr, flush=True)
+    acc = None
+    if not args.no_accuracy:
+        from oracle.vector import VectorSHTOracle
+        o = VectorSHTOracle.healpix(NSIDE, LMAX, MMAX)
+        f64 = f[:1].double().cpu().numpy()
+        a_ref = o.forward(f64)
+        g_ref = o.inverse(c0[:1].cpu().numpy().astype(np.complex128))
+        acc = {}
+        for prec, r in res.items():
+            acc[prec] = {"forward_rel_l2": float(np.linalg.norm(r["alm"] - a_ref) / np.linalg.norm(a_ref)),
+                         "inverse_rel_l2": float(np.linalg.norm(r["inv"] - g_ref) / np.linalg.norm(g_ref))}
+    main = res["bf16x3"]
+    print(json.dumps({
+        "metric": "fwd+inv vector SHT fields/s, nside=256 lmax=mmax=513, 32 (theta, phi) fields",
+        "value": main["fields_per_s"], "unit": "fields/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": main["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic (band-limited spheroidal + toroidal fields, C_l=(l+1)^-2, seeded)",
+        "config": {"workload": "VectorSHT + VectoriSHT, HEALPix nside=256, lmax=mmax=513, 32 vector fields",
+                   "precision": "bf16x3", "parallelism": "single GPU"},
+        "fp32_fields_per_s": res["fp32"]["fields_per_s"],
+        "speedup_vs_fp32": res["fp32"]["ms_per_step"] / main["ms_per_step"],
+        "tuned_shapes": {p: r["tuned_shapes"] for p, r in res.items()},
+        "plan_build_s": plan_s, "table_bytes": plan.table_bytes,
+        "accuracy_vs_fp64": acc}))
+
+
 MIXED_NSIDES = (64, 128, 256, 512)
 MIXED_FIELDS = 32
 
@@ -425,6 +499,8 @@ def main():
         return bench_mixed(args)
     if args.regrid:
         return bench_regrid(args)
+    if args.vector:
+        return bench_vector(args)
     t0 = time.perf_counter()
     plan = ShtPlan.healpix(NSIDE, LMAX, MMAX)
     torch.cuda.synchronize()
