@@ -339,6 +339,64 @@ struct StageArgs {
 // Wave-level kernels: work-group = one ring pair x WAVES fields
 // ---------------------------------------------------------------------------
 
+// Spectra of the WAVES fields of a work-group, in the LDS sequences
+// (lds + g * stride, point x at pad(x); the ring pair's DFT values), to the
+// folded spectra X[m][parity][k][2 b + re/im] (layout 0, all threads, fields
+// fastest so that consecutive lanes write consecutive columns) or to the
+// ring-major ftm (layout 1, one wave per field).
+template <int WAVES>
+__device__ __forceinline__ void analysis_store(const StageArgs& a, const RingPair& pr, int k, int64_t b0,
+                                               int64_t b, const float2* lds, int stride, const float2* phs,
+                                               float* __restrict__ X, int lane) {
+  constexpr int kThreadsW = WAVES * 64;
+  const int N = pr.nphi;
+  const float2* buf = lds + (threadIdx.x >> 6) * stride;
+  const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
+  const int K = min(a.mmax, N / 2 + 1);
+  if (a.layout == 0) {
+    // folded spectra X[m][parity][k][2b + re/im]: fields fastest so that
+    // consecutive lanes write consecutive columns
+    const int64_t pstride = (int64_t)a.kpad * a.ncol;
+    const int total = a.mmax * WAVES;
+    for (int e = threadIdx.x; e < total; e += kThreadsW) {
+      const int m = e / WAVES;
+      const int g = e - m * WAVES;
+      if (b0 + g >= a.nbatch) continue;
+      {
+        float2 S = make_float2(0.f, 0.f), D = S;
+        if (m < K) {
+          const float2* seq = lds + g * stride;
+          const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
+          float2 Fn, Fs;
+          unpack_pair(seq[pad(m)], seq[pad(m == 0 ? 0 : N - m)], phs[m], c, Fn, Fs);
+          if (pr.single) {
+            S = Fn;
+            D = Fn;
+          } else {
+            S = cadd(Fn, Fs);
+            D = csub(Fn, Fs);
+          }
+        }
+        float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * (b0 + g);
+        *reinterpret_cast<float2*>(base) = S;
+        *reinterpret_cast<float2*>(base + pstride) = D;
+      }
+    }
+  } else if (b < a.nbatch) {
+    // ring-major ftm[b][t][m]: lanes along m
+    float2* out = reinterpret_cast<float2*>(X) + b * a.nring * (int64_t)a.mmax;
+    for (int m = lane; m < a.mmax; m += 64) {
+      float2 Fn = make_float2(0.f, 0.f), Fs = Fn;
+      if (m < K) {
+        const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
+        unpack_pair(buf[pad(m)], buf[pad(m == 0 ? 0 : N - m)], phs[m], c, Fn, Fs);
+      }
+      out[(int64_t)k * a.mmax + m] = Fn;
+      if (!pr.single) out[(int64_t)(a.nring - 1 - k) * a.mmax + m] = Fs;
+    }
+  }
+}
+
 template <int WAVES, int LO, int HI>
 __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const float* __restrict__ f,
                                                                         float* __restrict__ X,
@@ -396,51 +454,8 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
     wave_sync();
     wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
   }
-  const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
-  const int K = min(a.mmax, N / 2 + 1);
   __syncthreads();
-  if (a.layout == 0) {
-    // folded spectra X[m][parity][k][2b + re/im]: fields fastest so that
-    // consecutive lanes write consecutive columns
-    const int64_t pstride = (int64_t)a.kpad * a.ncol;
-    const int total = a.mmax * WAVES;
-    for (int e = threadIdx.x; e < total; e += kThreadsW) {
-      const int m = e / WAVES;
-      const int g = e - m * WAVES;
-      if (b0 + g >= a.nbatch) continue;
-      {
-        float2 S = make_float2(0.f, 0.f), D = S;
-        if (m < K) {
-          const float2* seq = lds + g * stride;
-          const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
-          float2 Fn, Fs;
-          unpack_pair(seq[pad(m)], seq[pad(m == 0 ? 0 : N - m)], phs[m], c, Fn, Fs);
-          if (pr.single) {
-            S = Fn;
-            D = Fn;
-          } else {
-            S = cadd(Fn, Fs);
-            D = csub(Fn, Fs);
-          }
-        }
-        float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * (b0 + g);
-        *reinterpret_cast<float2*>(base) = S;
-        *reinterpret_cast<float2*>(base + pstride) = D;
-      }
-    }
-  } else if (b < a.nbatch) {
-    // ring-major ftm[b][t][m]: lanes along m
-    float2* out = reinterpret_cast<float2*>(X) + b * a.nring * (int64_t)a.mmax;
-    for (int m = lane; m < a.mmax; m += 64) {
-      float2 Fn = make_float2(0.f, 0.f), Fs = Fn;
-      if (m < K) {
-        const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
-        unpack_pair(buf[pad(m)], buf[pad(m == 0 ? 0 : N - m)], phs[m], c, Fn, Fs);
-      }
-      out[(int64_t)k * a.mmax + m] = Fn;
-      if (!pr.single) out[(int64_t)(a.nring - 1 - k) * a.mmax + m] = Fs;
-    }
-  }
+  analysis_store<WAVES>(a, pr, k, b0, b, lds, stride, phs, X, lane);
 }
 
 template <int WAVES, int LO, int HI>
