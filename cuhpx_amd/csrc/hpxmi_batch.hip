@@ -95,21 +95,24 @@ int autotune(ShtPlan& p, int precision, int64_t nbatch, float* ws, int64_t ws_by
     for (int c = 0; c < 3; ++c) {
       if (p.class_n[c] == 0) continue;
       float best = 1e30f;
-      for (int w : kFftWaveChoices[c]) {
-        float ms = 0.f;
-        const int rc = time_ms(
-            [&] {
-              return dir == 0 ? launch_fft_class(p, true, c, w, t.f, X, nbatch, 1, 1.f, 0, s, 0)
-                              : launch_fft_class(p, false, c, w, X, t.f, nbatch, 0, 1.f, 0, s, 0);
-            },
-            s, t.e0, t.e1, t.flush, &ms);
-        if (rc == HPX_ERR_UNSUPPORTED) continue;  // shape does not fit this plan's LDS
-        if (rc != HPX_OK) return rc;
-        if (ms < best) {
-          best = ms;
-          ft.waves[dir][c] = w;
+      for (int w : kFftWaveChoices[c])
+        for (int tpf : kFftTpfChoices) {
+          if (w * tpf > 1024) continue;
+          float ms = 0.f;
+          const int rc = time_ms(
+              [&] {
+                return dir == 0 ? launch_fft_class(p, true, c, w, tpf, t.f, X, nbatch, 1, 1.f, 0, s, 0)
+                                : launch_fft_class(p, false, c, w, tpf, X, t.f, nbatch, 0, 1.f, 0, s, 0);
+              },
+              s, t.e0, t.e1, t.flush, &ms);
+          if (rc == HPX_ERR_UNSUPPORTED) continue;  // shape does not fit this plan's LDS
+          if (rc != HPX_OK) return rc;
+          if (ms < best) {
+            best = ms;
+            ft.waves[dir][c] = w;
+            ft.tpf[dir][c] = tpf;
+          }
         }
-      }
     }
   GemmTune gt;
   float best_f = 1e30f, best_i = 1e30f;
@@ -200,7 +203,10 @@ extern "C" int hpx_sht_tuned_shapes(const hpx_sht_plan* plan, int precision, int
   const FftTune f = p->fft_tune(ncol);
   const GemmTune g = p->gemm_tune(precision, ncol);
   for (int d = 0; d < 2; ++d)
-    for (int c = 0; c < 3; ++c) shapes[3 * d + c] = f.waves[d][c];
+    for (int c = 0; c < 3; ++c) {
+      shapes[3 * d + c] = f.waves[d][c];
+      shapes[9 + 3 * d + c] = f.tpf[d][c];
+    }
   shapes[6] = g.fwd_ct;
   shapes[7] = g.inv_ct;
   shapes[8] = g.inv_rh;

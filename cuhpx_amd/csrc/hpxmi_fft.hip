@@ -33,8 +33,11 @@
 namespace hpxmi {
 namespace {
 
+// Explicit fused multiply-adds: the same rounding in every kernel shape (a
+// plain a*b - c*d may be contracted either way, differently per instantiation,
+// which would make results depend on the launch shape).
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -120,31 +123,43 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Barrier between the passes of one field's DFT: the field's own wavefront
+// (TPF = 64 threads per field), or the work-group when a field is spread over
+// several wavefronts.
+template <int TPF>
+__device__ __forceinline__ void field_sync() {
+  if constexpr (TPF == 64)
+    wave_sync();
+  else
+    __syncthreads();
+}
+
 // One radix-R Stockham pass over M = 2^LOG2M points by one wavefront, after
 // 2^LOG2NS points have been combined.  All strides are compile-time constants,
 // so the LDS accesses use immediate offsets.
-template <int R, int LOG2M, int LOG2NS>
+// TPF threads share the sequence (lane = thread index within the field).
+template <int R, int LOG2M, int LOG2NS, int TPF = 64>
 __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict__ tw, int log2_tab,
                                           int lane) {
   constexpr int kLogR = log2_of<R>();
   constexpr int kNB = 1 << (LOG2M - kLogR);  // butterflies; also the input stride
   constexpr int kNS = 1 << LOG2NS;
-  constexpr int kB = (kNB + 63) / 64;
+  constexpr int kB = (kNB + TPF - 1) / TPF;
   const int tw_shift = log2_tab - LOG2NS - kLogR;
   float2 v[kB][R];
   float2 w[kB];
 #pragma unroll
   for (int i = 0; i < kB; ++i) {
-    const int b = lane + 64 * i;
-    if (kNS > 1 && (kNB >= 64 || b < kNB)) w[i] = tw[(b & (kNS - 1)) << tw_shift];
+    const int b = lane + TPF * i;
+    if (kNS > 1 && (kNB >= TPF || b < kNB)) w[i] = tw[(b & (kNS - 1)) << tw_shift];
   }
   // padded positions: for power-of-two strides s, pad(base + r s) =
   // pad(base) + r s + ((r s) >> 4) when s is a multiple of 16 (reads) or base
   // is a Stockham output base (writes), so the offsets stay immediate
 #pragma unroll
   for (int i = 0; i < kB; ++i) {
-    const int b = lane + 64 * i;
-    if (kNB >= 64 || b < kNB) {
+    const int b = lane + TPF * i;
+    if (kNB >= TPF || b < kNB) {
       const int pb = pad(b);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
@@ -155,11 +170,11 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
       }
     }
   }
-  wave_sync();
+  field_sync<TPF>();
 #pragma unroll
   for (int i = 0; i < kB; ++i) {
-    const int b = lane + 64 * i;
-    if (kNB >= 64 || b < kNB) {
+    const int b = lane + TPF * i;
+    if (kNB >= TPF || b < kNB) {
       const int jn = b & (kNS - 1);
       if (kNS > 1) twiddle<R>(v[i], w[i]);
       dft<R>(v[i]);
@@ -168,33 +183,33 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
       for (int r = 0; r < R; ++r) buf[pbase + r * kNS + ((r * kNS) >> 4)] = v[i][r];
     }
   }
-  wave_sync();
+  field_sync<TPF>();
 }
 
 // Radix schedule: radix 4 for 2 or 4 remaining bits, radix 2 only for M = 2,
 // radix 8 otherwise.
-template <int LOG2M, int LOG2NS = 0>
+template <int LOG2M, int LOG2NS = 0, int TPF = 64>
 __device__ __forceinline__ void wave_fft_fixed(float2* buf, const float2* __restrict__ tw,
                                                int log2_tab, int lane) {
   if constexpr (LOG2NS < LOG2M) {
     constexpr int rem = LOG2M - LOG2NS;
     constexpr int R = rem == 1 ? 2 : ((rem == 2 || rem == 4) ? 4 : 8);
-    wave_pass<R, LOG2M, LOG2NS>(buf, tw, log2_tab, lane);
-    wave_fft_fixed<LOG2M, LOG2NS + log2_of<R>()>(buf, tw, log2_tab, lane);
+    wave_pass<R, LOG2M, LOG2NS, TPF>(buf, tw, log2_tab, lane);
+    wave_fft_fixed<LOG2M, LOG2NS + log2_of<R>(), TPF>(buf, tw, log2_tab, lane);
   }
 }
 
 // Forward DFT of 2^log2M points (natural order in and out) by one wavefront,
 // for the sizes of one size class (log2M in [LO, HI]).
-template <int LO, int HI>
+template <int LO, int HI, int TPF = 64>
 __device__ __forceinline__ void wave_fft(float2* buf, int log2M, const float2* __restrict__ tw,
                                          int log2_tab, int lane) {
   if constexpr (LO <= HI) {
     if (log2M == LO) {
-      wave_fft_fixed<LO>(buf, tw, log2_tab, lane);
+      wave_fft_fixed<LO, 0, TPF>(buf, tw, log2_tab, lane);
       return;
     }
-    wave_fft<LO + 1, HI>(buf, log2M, tw, log2_tab, lane);
+    wave_fft<LO + 1, HI, TPF>(buf, log2M, tw, log2_tab, lane);
   }
 }
 
@@ -204,70 +219,70 @@ __device__ __forceinline__ void wave_fft(float2* buf, int log2M, const float2* _
 // c = FFT(conj(FFT(a) * bhat)) with bhat = FFT(conj chirp) / M; the chirp
 // factors (N <= M/2 of them, lane-contiguous) are loaded before the second
 // FFT for the classes up to 2048 points, so their latency hides behind it.
-template <int LO, int HI>
+template <int LO, int HI, int TPF = 64>
 __device__ __forceinline__ void wave_ring_dft(float2* buf, const RingPair& pr,
                                               const float2* __restrict__ bhat,
                                               const float2* __restrict__ chirp,
                                               const float2* __restrict__ tw, int log2_tab, int lane) {
-  wave_fft<LO, HI>(buf, pr.log2M, tw, log2_tab, lane);
+  wave_fft<LO, HI, TPF>(buf, pr.log2M, tw, log2_tab, lane);
   if (pr.M != pr.nphi) {
-    constexpr int kItC = ((1 << HI) / 2 + 63) / 64;
+    constexpr int kItC = ((1 << HI) / 2 + TPF - 1) / TPF;
     constexpr bool kEarly = HI <= 11;
     float2 cw[kEarly ? kItC : 1];
     // all loads of the kernel spectrum in flight before the LDS updates
-    constexpr int kIt = ((1 << HI) + 63) / 64;
+    constexpr int kIt = ((1 << HI) + TPF - 1) / TPF;
     float2 bh[kIt];
 #pragma unroll
     for (int i = 0; i < kIt; ++i) {
-      const int x = lane + 64 * i;
+      const int x = lane + TPF * i;
       if (x < pr.M) bh[i] = bhat[x];
     }
 #pragma unroll
     for (int i = 0; i < kIt; ++i) {
-      const int x = lane + 64 * i;
-      const int px = pad(lane) + 68 * i;
+      const int x = lane + TPF * i;
+      const int px = pad(lane) + (TPF + TPF / 16) * i;
       if (x < pr.M) buf[px] = cconj(cmul(buf[px], bh[i]));
     }
     if constexpr (kEarly) {
 #pragma unroll
       for (int i = 0; i < kItC; ++i) {
-        const int x = lane + 64 * i;
+        const int x = lane + TPF * i;
         if (x < pr.nphi) cw[i] = chirp[x];
       }
     }
-    wave_sync();
-    wave_fft<LO, HI>(buf, pr.log2M, tw, log2_tab, lane);
+    field_sync<TPF>();
+    wave_fft<LO, HI, TPF>(buf, pr.log2M, tw, log2_tab, lane);
 #pragma unroll
     for (int i = 0; i < kItC; ++i) {
-      const int x = lane + 64 * i;
-      const int px = pad(lane) + 68 * i;
+      const int x = lane + TPF * i;
+      const int px = pad(lane) + (TPF + TPF / 16) * i;
       if (x < pr.nphi) {
         const float2 w = kEarly ? cw[kEarly ? i : 0] : chirp[x];
         buf[px] = cmul(w, cconj(buf[px]));
       }
     }
-    wave_sync();
+    field_sync<TPF>();
   }
 }
 
 // seq[x] *= chirp[x] for x < N (Bluestein pre-multiply of a packed sequence).
-template <int HI>
+template <int HI, int TPF = 64>
 __device__ __forceinline__ void chirp_premultiply(float2* buf, const float2* __restrict__ chirp, int N,
                                                   int lane) {
-  constexpr int kItC = ((1 << HI) / 2 + 63) / 64;
+  constexpr int kItC = ((1 << HI) / 2 + TPF - 1) / TPF;
   constexpr int kB = kItC < 8 ? kItC : 8;
 #pragma unroll
   for (int i0 = 0; i0 < kItC; i0 += kB) {
     float2 w[kB];
 #pragma unroll
     for (int u = 0; u < kB; ++u) {
-      const int x = lane + 64 * (i0 + u);
+      const int x = lane + TPF * (i0 + u);
       if (x < N) w[u] = chirp[x];
     }
 #pragma unroll
     for (int u = 0; u < kB; ++u) {
-      const int x = lane + 64 * (i0 + u);
-      const int px = pad(lane) + 68 * (i0 + u);
+      const int x = lane + TPF * (i0 + u);
+      const int px = pad(lane) + (TPF + TPF / 16) * (i0 + u);
       if (x < N) buf[px] = cmul(buf[px], w[u]);
     }
   }
@@ -344,13 +359,13 @@ struct StageArgs {
 // folded spectra X[m][parity][k][2 b + re/im] (layout 0, all threads, fields
 // fastest so that consecutive lanes write consecutive columns) or to the
 // ring-major ftm (layout 1, one wave per field).
-template <int WAVES>
+template <int WAVES, int TPF = 64>
 __device__ __forceinline__ void analysis_store(const StageArgs& a, const RingPair& pr, int k, int64_t b0,
                                                int64_t b, const float2* lds, int stride, const float2* phs,
                                                float* __restrict__ X, int lane) {
-  constexpr int kThreadsW = WAVES * 64;
+  constexpr int kThreadsW = WAVES * TPF;
   const int N = pr.nphi;
-  const float2* buf = lds + (threadIdx.x >> 6) * stride;
+  const float2* buf = lds + (threadIdx.x / TPF) * stride;
   const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
   if (a.layout == 0) {
@@ -383,9 +398,9 @@ __device__ __forceinline__ void analysis_store(const StageArgs& a, const RingPai
       }
     }
   } else if (b < a.nbatch) {
-    // ring-major ftm[b][t][m]: lanes along m
+    // ring-major ftm[b][t][m]: the field's threads along m
     float2* out = reinterpret_cast<float2*>(X) + b * a.nring * (int64_t)a.mmax;
-    for (int m = lane; m < a.mmax; m += 64) {
+    for (int m = lane; m < a.mmax; m += TPF) {
       float2 Fn = make_float2(0.f, 0.f), Fs = Fn;
       if (m < K) {
         const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
@@ -397,12 +412,13 @@ __device__ __forceinline__ void analysis_store(const StageArgs& a, const RingPai
   }
 }
 
-template <int WAVES, int LO, int HI>
-__global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const float* __restrict__ f,
-                                                                        float* __restrict__ X,
-                                                                        StageArgs a) {
+// WAVES fields per work-group, TPF threads (TPF / 64 wavefronts) per field.
+template <int WAVES, int LO, int HI, int TPF = 64>
+__global__ __launch_bounds__(WAVES * TPF) void ring_analysis_wave_kernel(const float* __restrict__ f,
+                                                                         float* __restrict__ X,
+                                                                         StageArgs a) {
   extern __shared__ float2 lds[];
-  constexpr int kThreadsW = WAVES * 64;
+  constexpr int kThreadsW = WAVES * TPF;
   const int64_t nfg = (a.nbatch + WAVES - 1) / WAVES;
   const int64_t item = xcd_local_index(blockIdx.x, gridDim.x);
   const int k = a.plist[item / nfg];
@@ -411,7 +427,7 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
   const int N = pr.nphi, M = pr.M;
   const bool blue = M != N;
   const float2* chirp = a.chirp + pr.chirp_off;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = threadIdx.x / TPF, lane = threadIdx.x % TPF;
   const int64_t b = b0 + wave;
   const int stride = wave_stride(M);
   float2* buf = lds + wave * stride;
@@ -421,10 +437,13 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
     const float2* ph = a.phase + (int64_t)k * a.mmax;
     for (int m = threadIdx.x; m < a.mmax; m += kThreadsW) phs[m] = ph[m];
   }
-  if (b < a.nbatch) {
-    const float* fn = f + b * a.npix + pr.off_n;
-    const float* fs = f + b * a.npix + (pr.single ? pr.off_n : pr.off_s);
-    constexpr int kIt = ((1 << HI) + 63) / 64;
+  // with several wavefronts per field the DFT has work-group barriers, so
+  // every field runs it (fields beyond the batch on zeros)
+  const bool active = b < a.nbatch;
+  if (active || TPF > 64) {
+    const float* fn = f + (active ? b : 0) * a.npix + pr.off_n;
+    const float* fs = f + (active ? b : 0) * a.npix + (pr.single ? pr.off_n : pr.off_s);
+    constexpr int kIt = ((1 << HI) + TPF - 1) / TPF;
     // all ring samples of the field in flight at once (one round trip)
     constexpr int kBatch = kIt < 32 ? kIt : 32;
 #pragma unroll
@@ -432,9 +451,9 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
       float2 z[kBatch], c[kBatch];
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        const int j = lane + 64 * (i0 + u);
+        const int j = lane + TPF * (i0 + u);
         z[u] = make_float2(0.f, 0.f);
-        if (j < N) {
+        if (active && j < N) {
           z[u].x = fn[j];
           z[u].y = fs[j];
           if (blue) c[u] = chirp[j];
@@ -442,28 +461,28 @@ __global__ __launch_bounds__(WAVES * 64) void ring_analysis_wave_kernel(const fl
       }
 #pragma unroll
       for (int u = 0; u < kBatch; ++u) {
-        const int j = lane + 64 * (i0 + u);
+        const int j = lane + TPF * (i0 + u);
         if (j < M) {
           float2 v = z[u];
           if (pr.single) v.y = 0.f;
           if (blue && j < N) v = cmul(v, c[u]);
-          buf[pad(lane) + 68 * (i0 + u)] = v;
+          buf[pad(lane) + (TPF + TPF / 16) * (i0 + u)] = v;
         }
       }
     }
-    wave_sync();
-    wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
+    field_sync<TPF>();
+    wave_ring_dft<LO, HI, TPF>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
   }
   __syncthreads();
-  analysis_store<WAVES>(a, pr, k, b0, b, lds, stride, phs, X, lane);
+  analysis_store<WAVES, TPF>(a, pr, k, b0, b, lds, stride, phs, X, lane);
 }
 
-template <int WAVES, int LO, int HI>
-__global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const float* __restrict__ X,
-                                                                         float* __restrict__ f,
-                                                                         StageArgs a) {
+template <int WAVES, int LO, int HI, int TPF = 64>
+__global__ __launch_bounds__(WAVES * TPF) void ring_synthesis_wave_kernel(const float* __restrict__ X,
+                                                                          float* __restrict__ f,
+                                                                          StageArgs a) {
   extern __shared__ float2 lds[];
-  constexpr int kThreadsW = WAVES * 64;
+  constexpr int kThreadsW = WAVES * TPF;
   const int64_t nfg = (a.nbatch + WAVES - 1) / WAVES;
   const int64_t item = xcd_local_index(blockIdx.x, gridDim.x);
   const int k = a.plist[item / nfg];
@@ -471,7 +490,7 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
   const RingPair pr = a.pairs[k];
   const int N = pr.nphi, M = pr.M;
   const float2* chirp = a.chirp + pr.chirp_off;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wave = threadIdx.x / TPF, lane = threadIdx.x % TPF;
   const int64_t b = b0 + wave;
   const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
@@ -539,20 +558,23 @@ __global__ __launch_bounds__(WAVES * 64) void ring_synthesis_wave_kernel(const f
     }
   }
   __syncthreads();
-  if (b >= a.nbatch) return;
+  // with several wavefronts per field the DFT has work-group barriers, so
+  // every field runs it (fields beyond the batch on zeros)
+  if (TPF == 64 && b >= a.nbatch) return;
   float2* buf = lds + wave * stride;
   if (M != N) {
-    chirp_premultiply<HI>(buf, chirp, N, lane);
-    wave_sync();
+    chirp_premultiply<HI, TPF>(buf, chirp, N, lane);
+    field_sync<TPF>();
   }
-  wave_ring_dft<LO, HI>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
+  wave_ring_dft<LO, HI, TPF>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
+  if (b >= a.nbatch) return;
   float* fr = f + b * a.npix;
-  constexpr int kIt = ((1 << HI) + 63) / 64;
+  constexpr int kIt = ((1 << HI) + TPF - 1) / TPF;
 #pragma unroll
   for (int u = 0; u < kIt; ++u) {
-    const int j = lane + 64 * u;
+    const int j = lane + TPF * u;
     if (j < N) {
-      const float2 r = buf[pad(lane) + 68 * u];
+      const float2 r = buf[pad(lane) + (TPF + TPF / 16) * u];
       fr[pr.off_n + j] = r.x;
       if (!pr.single) fr[pr.off_s + j] = -r.y;
     }
@@ -750,13 +772,13 @@ StageArgs make_args(const ShtPlan& p, int64_t nbatch, int use_weight, float scal
   return a;
 }
 
-template <int WAVES, int C>
+template <int WAVES, int C, int TPF = 64>
 void launch_wave(bool analysis, unsigned grid, size_t lds_bytes, const float* in, float* out,
                  const StageArgs& a, hipStream_t s) {
   constexpr int kLog2 = 10 + C;
   constexpr int kLo = C == 0 ? 1 : kLog2;
-  auto ka = ring_analysis_wave_kernel<WAVES, kLo, kLog2>;
-  auto ks = ring_synthesis_wave_kernel<WAVES, kLo, kLog2>;
+  auto ka = ring_analysis_wave_kernel<WAVES, kLo, kLog2, TPF>;
+  auto ks = ring_synthesis_wave_kernel<WAVES, kLo, kLog2, TPF>;
   // the padded sequences take a little more than 64 KB per work-group
   // (gfx950 allows 160 KB)
   static const bool attr_set = [&] {
@@ -768,14 +790,14 @@ void launch_wave(bool analysis, unsigned grid, size_t lds_bytes, const float* in
   }();
   (void)attr_set;
   if (analysis)
-    ka<<<grid, WAVES * 64, lds_bytes, s>>>(in, out, a);
+    ka<<<grid, WAVES * TPF, lds_bytes, s>>>(in, out, a);
   else
-    ks<<<grid, WAVES * 64, lds_bytes, s>>>(in, out, a);
+    ks<<<grid, WAVES * TPF, lds_bytes, s>>>(in, out, a);
 }
 
 }  // namespace
 
-int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, const float* in,
+int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, int tpf, const float* in,
                      float* out, int64_t nbatch, int use_weight, float scale, int cmode,
                      hipStream_t s, int layout) {
   const int n = p.class_n[c];
@@ -792,26 +814,26 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, const fl
     HPX_LAUNCH_CHECK();
     return HPX_OK;
   }
-  // class c < 3: FFT length <= 2^(10 + c); one field per wavefront
+  // class c < 3: FFT length <= 2^(10 + c); tpf threads per field
   if (waves != kFftWaveChoices[c][0] && waves != kFftWaveChoices[c][1] && waves != kFftWaveChoices[c][2])
-    HPX_FAIL(HPX_ERR_ARG, "no FFT kernel with %d waves for size class %d", waves, c);
+    HPX_FAIL(HPX_ERR_ARG, "no FFT kernel with %d fields per work-group for size class %d", waves, c);
+  if ((tpf != 64 && tpf != 128) || waves * tpf > 1024)
+    HPX_FAIL(HPX_ERR_ARG, "no FFT kernel with %d fields of %d threads", waves, tpf);
   const int64_t nfg = (nbatch + waves - 1) / waves;
   const int64_t items = (int64_t)n * nfg;
   if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   const size_t lds_bytes = ((size_t)waves * wave_stride(1 << (10 + c)) + p.mmax) * sizeof(float2);
   if (lds_bytes > 160 * 1024) HPX_FAIL(HPX_ERR_UNSUPPORTED, "mmax too large for the ring FFT kernels");
   const unsigned grid = (unsigned)items;
-  switch (c * 32 + waves) {
-    case 0 * 32 + 4: launch_wave<4, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 0 * 32 + 8: launch_wave<8, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 0 * 32 + 16: launch_wave<16, 0>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 1 * 32 + 2: launch_wave<2, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 1 * 32 + 4: launch_wave<4, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 1 * 32 + 8: launch_wave<8, 1>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 2 * 32 + 1: launch_wave<1, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
-    case 2 * 32 + 2: launch_wave<2, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
-    default: launch_wave<4, 2>(analysis, grid, lds_bytes, in, out, a, s); break;
+#define HPX_W(W, C, T) \
+  case (C * 32 + W) * 2 + (T == 128): launch_wave<W, C, T>(analysis, grid, lds_bytes, in, out, a, s); break;
+  switch ((c * 32 + waves) * 2 + (tpf == 128)) {
+    HPX_W(4, 0, 64) HPX_W(8, 0, 64) HPX_W(16, 0, 64) HPX_W(4, 0, 128) HPX_W(8, 0, 128)
+    HPX_W(2, 1, 64) HPX_W(4, 1, 64) HPX_W(8, 1, 64) HPX_W(2, 1, 128) HPX_W(4, 1, 128) HPX_W(8, 1, 128)
+    HPX_W(1, 2, 64) HPX_W(2, 2, 64) HPX_W(4, 2, 64) HPX_W(1, 2, 128) HPX_W(2, 2, 128) HPX_W(4, 2, 128)
+    default: HPX_FAIL(HPX_ERR_ARG, "no FFT kernel for class %d, %d fields, %d threads", c, waves, tpf);
   }
+#undef HPX_W
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
@@ -829,7 +851,8 @@ int launch_stage(const ShtPlan& p, const float* in, float* out, int64_t nbatch, 
   const FftTune t = p.fft_tune(col_pad(nbatch));
   for (int c = 0; c < kFftClasses; ++c) {
     const int waves = c < 3 ? t.waves[ANALYSIS ? 0 : 1][c] : 1;
-    const int rc = launch_fft_class(p, ANALYSIS, c, waves, in, out, nbatch, use_weight, scale, cmode, s,
+    const int tpf = c < 3 ? t.tpf[ANALYSIS ? 0 : 1][c] : 64;
+    const int rc = launch_fft_class(p, ANALYSIS, c, waves, tpf, in, out, nbatch, use_weight, scale, cmode, s,
                                     layout);
     if (rc != HPX_OK) return rc;
   }

@@ -14,7 +14,8 @@ namespace hpxmi {
 // wavefront) per work-group for the wave-level size classes 0..2, for the
 // analysis [0] and synthesis [1] directions.
 struct FftTune {
-  int waves[2][3] = {{16, 8, 2}, {16, 8, 2}};
+  int waves[2][3] = {{16, 8, 2}, {16, 8, 2}};       // fields per work-group [direction][class]
+  int tpf[2][3] = {{64, 128, 64}, {64, 128, 64}};   // threads per field (64 or 128)
 };
 // Launch shapes of the Legendre GEMMs for one (precision, column count):
 // 16-column MFMA tiles per wavefront (work-group column block = 64 * ct).
@@ -25,6 +26,9 @@ struct GemmTune {
 };
 // Allowed values (what the kernels are instantiated for).
 constexpr int kFftWaveChoices[3][3] = {{4, 8, 16}, {2, 4, 8}, {1, 2, 4}};
+// Threads per field of the FFT kernels: one wavefront, or two sharing each
+// pass (a work-group barrier between passes); fields * threads <= 1024.
+constexpr int kFftTpfChoices[2] = {64, 128};
 constexpr int kGemmCtChoices[2] = {1, 2};
 constexpr int kGemmRhChoices[2] = {1, 2};
 
@@ -137,7 +141,7 @@ int launch_analysis(const ShtPlan& p, const float* f, float* X, int64_t nbatch,
 int launch_synthesis(const ShtPlan& p, const float* X, float* f, int64_t nbatch,
                      int use_weight, float scale, int cmode, hipStream_t s, int layout = 0);
 // One size class of the ring FFT stage with an explicit work-group shape.
-int launch_fft_class(const ShtPlan& p, bool analysis, int cls, int waves, const float* in,
+int launch_fft_class(const ShtPlan& p, bool analysis, int cls, int waves, int tpf, const float* in,
                      float* out, int64_t nbatch, int use_weight, float scale, int cmode,
                      hipStream_t s, int layout);
 // The GEMMs alone (X <-> aT) with an explicit column tiling, and the

@@ -151,12 +151,13 @@ class ShtPlan:
 
     def tuned_shapes(self, precision, nbatch):
         """Launch shapes used for (precision, nbatch)."""
-        out = (ctypes.c_int32 * 9)()
+        out = (ctypes.c_int32 * 15)()
         rc = _lib.lib().hpx_sht_tuned_shapes(self._handle, _lib.PRECISIONS[precision], nbatch, out)
         _lib.check(rc, "SHT tuned shapes")
         v = list(out)
         return {"fft_analysis_waves": v[0:3], "fft_synthesis_waves": v[3:6], "gemm_fwd_col_tiles": v[6],
-                "gemm_inv_col_tiles": v[7], "gemm_inv_row_blocks": v[8]}
+                "gemm_inv_col_tiles": v[7], "gemm_inv_row_blocks": v[8],
+                "fft_analysis_threads_per_field": v[9:12], "fft_synthesis_threads_per_field": v[12:15]}
 
     def _run(self, fn, *args, nbatch):
         ws, nbytes = self.workspace(nbatch)

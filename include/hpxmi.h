@@ -253,10 +253,11 @@ int hpx_sht_stage_ring_synthesis(const hpx_sht_plan* plan, float* f, int64_t nba
 int hpx_sht_autotune(hpx_sht_plan* plan, int precision, int64_t nbatch, void* workspace,
                      int64_t workspace_bytes, hipStream_t stream);
 
-/* The launch shapes used for (precision, nbatch), 9 values: shapes[0..2]
+/* The launch shapes used for (precision, nbatch), 15 values: shapes[0..2]
  * analysis fields per work-group of FFT classes 0..2, shapes[3..5] the same
  * for synthesis, shapes[6] / shapes[7] forward / inverse GEMM column tiles,
- * shapes[8] inverse GEMM ring-pair rows per work-group in units of 64. */
+ * shapes[8] inverse GEMM ring-pair rows per work-group in units of 64,
+ * shapes[9..14] threads per field (64 or 128) of the six FFT entries 0..5. */
 int hpx_sht_tuned_shapes(const hpx_sht_plan* plan, int precision, int64_t nbatch,
                          int32_t* shapes);
 

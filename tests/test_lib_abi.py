@@ -119,5 +119,5 @@ def test_mixed_and_autotune_reject_bad_arguments():
     assert lib.hpx_sht_mixed_launch(None, None) == _lib.HPX_ERR_ARG
     lib.hpx_sht_mixed_destroy(None)
     assert lib.hpx_sht_autotune(None, 1, 4, None, 0, None) == _lib.HPX_ERR_ARG
-    shapes = (ctypes.c_int32 * 9)()
+    shapes = (ctypes.c_int32 * 15)()
     assert lib.hpx_sht_tuned_shapes(None, 1, 4, shapes) == _lib.HPX_ERR_ARG
