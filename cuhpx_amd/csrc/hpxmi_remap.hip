@@ -165,10 +165,10 @@ __device__ __forceinline__ void element(const Side& side, const TileGeom& g, int
 // TBMAX: largest tile side (64 or 128; the work-group has TBMAX^2 / 16
 // threads).  Ring segments in a tile average TB/2 pixels, so the RING kinds
 // use 128-pixel tiles (two 64 KB LDS buffers for 4-byte pixels) to write or
-// read longer runs.  NT: non-temporal row loads and stores, which help when
-// both sides stream whole rows of the tile (NEST <-> flat) and hurt the
-// partial-line runs of the RING side, which the L2 has to merge.
-template <typename T, int TBMAX, bool NT>
+// read longer runs.  NTL / NTS: non-temporal loads / stores, for a side that
+// streams whole lines of the tile (NEST or flat); the partial-line runs of a
+// RING side go through the L2, which merges them.
+template <typename T, int TBMAX, bool NTL, bool NTS>
 __global__ __launch_bounds__(TBMAX * TBMAX / kPerThread) void remap_tiles(
     const T* __restrict__ in, T* __restrict__ out, int64_t nside, int order,
     int tb, int log_tb, int tiles_per_face, int64_t nrows, int rows_per_chunk,
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(TBMAX * TBMAX / kPerThread) void remap_tiles(
     T v[kPerThread];
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i)
-      if (src_q[i] >= 0) v[i] = NT ? __builtin_nontemporal_load(src_row + src_i[i]) : src_row[src_i[i]];
+      if (src_q[i] >= 0) v[i] = NTL ? __builtin_nontemporal_load(src_row + src_i[i]) : src_row[src_i[i]];
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i)
       if (src_q[i] >= 0) lds[buf][src_q[i]] = v[i];
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(TBMAX * TBMAX / kPerThread) void remap_tiles(
 #pragma unroll
     for (int i = 0; i < kPerThread; ++i)
       if (dst_q[i] >= 0) {
-        if constexpr (NT)
+        if constexpr (NTS)
           __builtin_nontemporal_store(lds[buf][dst_q[i]], dst_row + dst_i[i]);
         else
           dst_row[dst_i[i]] = lds[buf][dst_q[i]];
@@ -278,7 +278,7 @@ Side flat_side(int origin, int cw) {
   return s;
 }
 
-template <typename T, int TBMAX, bool NT>
+template <typename T, int TBMAX, bool NTL, bool NTS>
 int launch_tiles(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Side dst,
                  hipStream_t stream) {
   constexpr int kThreads = TBMAX * TBMAX / kPerThread;
@@ -297,7 +297,7 @@ int launch_tiles(const void* in, void* out, int64_t nside, int64_t nrows, Side s
   const int rows_per_chunk = (int)((nrows + chunks - 1) / chunks);
   chunks = (nrows + rows_per_chunk - 1) / rows_per_chunk;
   dim3 grid(ntiles, (unsigned)chunks);
-  remap_tiles<T, TBMAX, NT><<<grid, kThreads, 0, stream>>>(
+  remap_tiles<T, TBMAX, NTL, NTS><<<grid, kThreads, 0, stream>>>(
       static_cast<const T*>(in), static_cast<T*>(out), nside, order, tb, log_tb,
       tiles_per_face, nrows, rows_per_chunk, src, dst);
   HPX_LAUNCH_CHECK();
@@ -310,10 +310,16 @@ int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Si
   const bool ring = src.scheme == kRing || dst.scheme == kRing;
   const bool nest_flat = (src.scheme == kNest && dst.scheme == kFlat) || (src.scheme == kFlat && dst.scheme == kNest);
   if constexpr (sizeof(T) <= 4) {
-    if (ring && nside >= 128) return launch_tiles<T, 128, false>(in, out, nside, nrows, src, dst, stream);
+    if (ring && nside >= 128) {
+      // RING -> NEST / flat: the destination streams whole lines, stored
+      // non-temporally (ring2nest 1.33 -> 1.25 ms at nside 1024, 64 rows);
+      // non-temporal loads of a NEST / flat source measured slower
+      if (src.scheme == kRing) return launch_tiles<T, 128, false, true>(in, out, nside, nrows, src, dst, stream);
+      return launch_tiles<T, 128, false, false>(in, out, nside, nrows, src, dst, stream);
+    }
   }
-  if (nest_flat) return launch_tiles<T, 64, true>(in, out, nside, nrows, src, dst, stream);
-  return launch_tiles<T, 64, false>(in, out, nside, nrows, src, dst, stream);
+  if (nest_flat) return launch_tiles<T, 64, true, true>(in, out, nside, nrows, src, dst, stream);
+  return launch_tiles<T, 64, false, false>(in, out, nside, nrows, src, dst, stream);
 }
 
 }  // namespace
