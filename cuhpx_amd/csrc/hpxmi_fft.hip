@@ -817,7 +817,7 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, int tpf,
   // class c < 3: FFT length <= 2^(10 + c); tpf threads per field
   if (waves != kFftWaveChoices[c][0] && waves != kFftWaveChoices[c][1] && waves != kFftWaveChoices[c][2])
     HPX_FAIL(HPX_ERR_ARG, "no FFT kernel with %d fields per work-group for size class %d", waves, c);
-  if ((tpf != 64 && tpf != 128) || waves * tpf > 1024)
+  if ((tpf != 64 && tpf != 128 && tpf != 256) || waves * tpf > 1024)
     HPX_FAIL(HPX_ERR_ARG, "no FFT kernel with %d fields of %d threads", waves, tpf);
   const int64_t nfg = (nbatch + waves - 1) / waves;
   const int64_t items = (int64_t)n * nfg;
@@ -826,11 +826,13 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, int tpf,
   if (lds_bytes > 160 * 1024) HPX_FAIL(HPX_ERR_UNSUPPORTED, "mmax too large for the ring FFT kernels");
   const unsigned grid = (unsigned)items;
 #define HPX_W(W, C, T) \
-  case (C * 32 + W) * 2 + (T == 128): launch_wave<W, C, T>(analysis, grid, lds_bytes, in, out, a, s); break;
-  switch ((c * 32 + waves) * 2 + (tpf == 128)) {
-    HPX_W(4, 0, 64) HPX_W(8, 0, 64) HPX_W(16, 0, 64) HPX_W(4, 0, 128) HPX_W(8, 0, 128)
+  case (C * 32 + W) * 8 + T / 64: launch_wave<W, C, T>(analysis, grid, lds_bytes, in, out, a, s); break;
+  switch ((c * 32 + waves) * 8 + tpf / 64) {
+    HPX_W(4, 0, 64) HPX_W(8, 0, 64) HPX_W(16, 0, 64) HPX_W(4, 0, 128) HPX_W(8, 0, 128) HPX_W(4, 0, 256)
     HPX_W(2, 1, 64) HPX_W(4, 1, 64) HPX_W(8, 1, 64) HPX_W(2, 1, 128) HPX_W(4, 1, 128) HPX_W(8, 1, 128)
+    HPX_W(2, 1, 256) HPX_W(4, 1, 256)
     HPX_W(1, 2, 64) HPX_W(2, 2, 64) HPX_W(4, 2, 64) HPX_W(1, 2, 128) HPX_W(2, 2, 128) HPX_W(4, 2, 128)
+    HPX_W(1, 2, 256) HPX_W(2, 2, 256) HPX_W(4, 2, 256)
     default: HPX_FAIL(HPX_ERR_ARG, "no FFT kernel for class %d, %d fields, %d threads", c, waves, tpf);
   }
 #undef HPX_W

@@ -15,7 +15,7 @@ namespace hpxmi {
 // analysis [0] and synthesis [1] directions.
 struct FftTune {
   int waves[2][3] = {{16, 8, 2}, {16, 8, 2}};       // fields per work-group [direction][class]
-  int tpf[2][3] = {{64, 128, 64}, {64, 128, 64}};   // threads per field (64 or 128)
+  int tpf[2][3] = {{64, 128, 64}, {64, 128, 64}};   // threads per field (64, 128 or 256)
 };
 // Launch shapes of the Legendre GEMMs for one (precision, column count):
 // 16-column MFMA tiles per wavefront (work-group column block = 64 * ct).
@@ -26,9 +26,9 @@ struct GemmTune {
 };
 // Allowed values (what the kernels are instantiated for).
 constexpr int kFftWaveChoices[3][3] = {{4, 8, 16}, {2, 4, 8}, {1, 2, 4}};
-// Threads per field of the FFT kernels: one wavefront, or two sharing each
-// pass (a work-group barrier between passes); fields * threads <= 1024.
-constexpr int kFftTpfChoices[2] = {64, 128};
+// Threads per field of the FFT kernels: one wavefront, or two or four sharing
+// each pass (a work-group barrier between passes); fields * threads <= 1024.
+constexpr int kFftTpfChoices[3] = {64, 128, 256};
 constexpr int kGemmCtChoices[2] = {1, 2};
 constexpr int kGemmRhChoices[2] = {1, 2};
 

@@ -257,7 +257,7 @@ int hpx_sht_autotune(hpx_sht_plan* plan, int precision, int64_t nbatch, void* wo
  * analysis fields per work-group of FFT classes 0..2, shapes[3..5] the same
  * for synthesis, shapes[6] / shapes[7] forward / inverse GEMM column tiles,
  * shapes[8] inverse GEMM ring-pair rows per work-group in units of 64,
- * shapes[9..14] threads per field (64 or 128) of the six FFT entries 0..5. */
+ * shapes[9..14] threads per field (64, 128 or 256) of the six FFT entries 0..5. */
 int hpx_sht_tuned_shapes(const hpx_sht_plan* plan, int precision, int64_t nbatch,
                          int32_t* shapes);
 
