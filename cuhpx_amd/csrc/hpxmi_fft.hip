@@ -94,19 +94,60 @@ __device__ __forceinline__ void dft<8>(float2* v) {
   }
 }
 
-template <int R>
-constexpr int log2_of() {
-  return R == 8 ? 3 : (R == 4 ? 2 : 1);
+template <>
+__device__ __forceinline__ void dft<16>(float2* v) {
+  float2 e[8], o[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    e[q] = v[2 * q];
+    o[q] = v[2 * q + 1];
+  }
+  dft<8>(e);
+  dft<8>(o);
+  // o[k] *= exp(-2 pi i k / 16)
+  const float c1 = 0.923879533f, s1 = 0.382683432f, h = 0.70710678118654752f;
+  o[1] = cmul(o[1], make_float2(c1, -s1));
+  o[2] = make_float2((o[2].x + o[2].y) * h, (o[2].y - o[2].x) * h);
+  o[3] = cmul(o[3], make_float2(s1, -c1));
+  o[4] = cmul_mi(o[4]);
+  o[5] = cmul(o[5], make_float2(-s1, -c1));
+  o[6] = make_float2((o[6].y - o[6].x) * h, -(o[6].x + o[6].y) * h);
+  o[7] = cmul(o[7], make_float2(-c1, -s1));
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    v[k] = cadd(e[k], o[k]);
+    v[k + 8] = csub(e[k], o[k]);
+  }
 }
 
-// v[r] *= w^r for r = 1..R-1 (powers of the table entry by multiplication).
+template <int R>
+constexpr int log2_of() {
+  return R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
+}
+
+// v[r] *= w^r for r = 1..R-1, powers of the table entry by multiplication:
+// sequentially up to R = 8; for R = 16 as w^r = w^(r & (r-1)) w^(r & -r)
+// from the squares w^2, w^4, w^8 (at most 6 roundings deep instead of 14).
 template <int R>
 __device__ __forceinline__ void twiddle(float2* v, float2 w1) {
-  float2 w = w1;
+  if constexpr (R == 16) {
+    float2 pw[16];
+    pw[1] = w1;
+    pw[2] = cmul(w1, w1);
+    pw[4] = cmul(pw[2], pw[2]);
+    pw[8] = cmul(pw[4], pw[4]);
 #pragma unroll
-  for (int r = 1; r < R; ++r) {
-    v[r] = cmul(v[r], w);
-    if (r + 1 < R) w = cmul(w, w1);
+    for (int r = 3; r < 16; ++r)
+      if (r & (r - 1)) pw[r] = cmul(pw[r & (r - 1)], pw[r & -r]);
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], pw[r]);
+  } else {
+    float2 w = w1;
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+      v[r] = cmul(v[r], w);
+      if (r + 1 < R) w = cmul(w, w1);
+    }
   }
 }
 
@@ -186,14 +227,15 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
   field_sync<TPF>();
 }
 
-// Radix schedule: radix 4 for 2 or 4 remaining bits, radix 2 only for M = 2,
-// radix 8 otherwise.
+// Radix schedule: radix 16 while at least 6 bits remain or exactly 4, then
+// 8 for 3 or 5 remaining bits, 4 for 2, 2 for 1 (1024 and 2048 points in 3
+// passes, 4096 in 3).
 template <int LOG2M, int LOG2NS = 0, int TPF = 64>
 __device__ __forceinline__ void wave_fft_fixed(float2* buf, const float2* __restrict__ tw,
                                                int log2_tab, int lane) {
   if constexpr (LOG2NS < LOG2M) {
     constexpr int rem = LOG2M - LOG2NS;
-    constexpr int R = rem == 1 ? 2 : ((rem == 2 || rem == 4) ? 4 : 8);
+    constexpr int R = (rem >= 6 || rem == 4) ? 16 : ((rem == 3 || rem == 5) ? 8 : (rem == 2 ? 4 : 2));
     wave_pass<R, LOG2M, LOG2NS, TPF>(buf, tw, log2_tab, lane);
     wave_fft_fixed<LOG2M, LOG2NS + log2_of<R>(), TPF>(buf, tw, log2_tab, lane);
   }
