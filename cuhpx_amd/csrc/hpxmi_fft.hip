@@ -179,13 +179,21 @@ __device__ __forceinline__ void field_sync() {
 // 2^LOG2NS points have been combined.  All strides are compile-time constants,
 // so the LDS accesses use immediate offsets.
 // TPF threads share the sequence (lane = thread index within the field).
-template <int R, int LOG2M, int LOG2NS, int TPF = 64>
+// Pointwise factors fused into the pass (the Bluestein steps):
+//   PRO = 1 (first pass): input x *= pro[x] for x < n (chirp pre-multiply);
+//   EPI = 1 (last pass): output x = conj(output x * epi[x]) (kernel spectrum);
+//   EPI = 2 (last pass): output x = epi[x] conj(output x) for x < n (chirp).
+// In the first pass input x = b + r kNB; in the last pass output x = b + r kNS.
+template <int R, int LOG2M, int LOG2NS, int TPF = 64, int PRO = 0, int EPI = 0>
 __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict__ tw, int log2_tab,
-                                          int lane) {
+                                          int lane, const float2* __restrict__ pro = nullptr,
+                                          const float2* __restrict__ epi = nullptr, int n = 0) {
   constexpr int kLogR = log2_of<R>();
   constexpr int kNB = 1 << (LOG2M - kLogR);  // butterflies; also the input stride
   constexpr int kNS = 1 << LOG2NS;
   constexpr int kB = (kNB + TPF - 1) / TPF;
+  static_assert(PRO == 0 || LOG2NS == 0, "pre-multiply on the first pass only");
+  static_assert(EPI == 0 || LOG2NS + kLogR == LOG2M, "post-multiply on the last pass only");
   const int tw_shift = log2_tab - LOG2NS - kLogR;
   float2 v[kB][R];
   float2 w[kB];
@@ -211,6 +219,32 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
       }
     }
   }
+  // the pointwise factors, requested while the LDS reads complete
+  float2 f[EPI ? kB : (PRO ? kB : 1)][(EPI || PRO) ? R : 1];
+  if constexpr (PRO || EPI) {
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int b = lane + TPF * i;
+      if (kNB >= TPF || b < kNB) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int x = PRO ? b + r * kNB : b + r * kNS;
+          if (EPI == 1 || x < n) f[i][r] = PRO ? pro[x] : epi[x];
+        }
+      }
+    }
+  }
+  if constexpr (PRO) {
+#pragma unroll
+    for (int i = 0; i < kB; ++i) {
+      const int b = lane + TPF * i;
+      if (kNB >= TPF || b < kNB) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (b + r * kNB < n) v[i][r] = cmul(v[i][r], f[i][r]);
+      }
+    }
+  }
   field_sync<TPF>();
 #pragma unroll
   for (int i = 0; i < kB; ++i) {
@@ -219,6 +253,14 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
       const int jn = b & (kNS - 1);
       if (kNS > 1) twiddle<R>(v[i], w[i]);
       dft<R>(v[i]);
+      if constexpr (EPI == 1) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[i][r] = cconj(cmul(v[i][r], f[i][r]));
+      } else if constexpr (EPI == 2) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (b + r * kNS < n) v[i][r] = cmul(f[i][r], cconj(v[i][r]));
+      }
       const int pbase = pad((b - jn) * R + jn);
 #pragma unroll
       for (int r = 0; r < R; ++r) buf[pbase + r * kNS + ((r * kNS) >> 4)] = v[i][r];
@@ -230,104 +272,51 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
 // Radix schedule: radix 16 while at least 6 bits remain or exactly 4, then
 // 8 for 3 or 5 remaining bits, 4 for 2, 2 for 1 (1024 and 2048 points in 3
 // passes, 4096 in 3).
-template <int LOG2M, int LOG2NS = 0, int TPF = 64>
+template <int LOG2M, int LOG2NS = 0, int TPF = 64, int PRO = 0, int EPI = 0>
 __device__ __forceinline__ void wave_fft_fixed(float2* buf, const float2* __restrict__ tw,
-                                               int log2_tab, int lane) {
+                                               int log2_tab, int lane, const float2* __restrict__ pro = nullptr,
+                                               const float2* __restrict__ epi = nullptr, int n = 0) {
   if constexpr (LOG2NS < LOG2M) {
     constexpr int rem = LOG2M - LOG2NS;
     constexpr int R = (rem >= 6 || rem == 4) ? 16 : ((rem == 3 || rem == 5) ? 8 : (rem == 2 ? 4 : 2));
-    wave_pass<R, LOG2M, LOG2NS, TPF>(buf, tw, log2_tab, lane);
-    wave_fft_fixed<LOG2M, LOG2NS + log2_of<R>(), TPF>(buf, tw, log2_tab, lane);
+    constexpr bool first = LOG2NS == 0, last = LOG2NS + log2_of<R>() == LOG2M;
+    wave_pass<R, LOG2M, LOG2NS, TPF, first ? PRO : 0, last ? EPI : 0>(buf, tw, log2_tab, lane, pro, epi, n);
+    wave_fft_fixed<LOG2M, LOG2NS + log2_of<R>(), TPF, PRO, EPI>(buf, tw, log2_tab, lane, pro, epi, n);
   }
 }
 
 // Forward DFT of 2^log2M points (natural order in and out) by one wavefront,
 // for the sizes of one size class (log2M in [LO, HI]).
-template <int LO, int HI, int TPF = 64>
+template <int LO, int HI, int TPF = 64, int PRO = 0, int EPI = 0>
 __device__ __forceinline__ void wave_fft(float2* buf, int log2M, const float2* __restrict__ tw,
-                                         int log2_tab, int lane) {
+                                         int log2_tab, int lane, const float2* __restrict__ pro = nullptr,
+                                         const float2* __restrict__ epi = nullptr, int n = 0) {
   if constexpr (LO <= HI) {
     if (log2M == LO) {
-      wave_fft_fixed<LO, 0, TPF>(buf, tw, log2_tab, lane);
+      wave_fft_fixed<LO, 0, TPF, PRO, EPI>(buf, tw, log2_tab, lane, pro, epi, n);
       return;
     }
-    wave_fft<LO + 1, HI, TPF>(buf, log2M, tw, log2_tab, lane);
+    wave_fft<LO + 1, HI, TPF, PRO, EPI>(buf, log2M, tw, log2_tab, lane, pro, epi, n);
   }
 }
 
-// DFT of length N of the wave's sequence (entered as z_j, or z_j * chirp_j
-// zero-padded to M for Bluestein), in place: afterwards point x < N of the
-// sequence holds the DFT value at x.  Bluestein: X[x] = chirp_x conj(c_x),
-// c = FFT(conj(FFT(a) * bhat)) with bhat = FFT(conj chirp) / M; the chirp
-// factors (N <= M/2 of them, lane-contiguous) are loaded before the second
-// FFT for the classes up to 2048 points, so their latency hides behind it.
-template <int LO, int HI, int TPF = 64>
+// DFT of length N of the wave's sequence (entered as z_j, or for Bluestein
+// z_j * chirp_j zero-padded to M -- PRECHIRP: z_j, multiplied by chirp_j in
+// the first pass), in place: afterwards point x < N of the sequence holds the
+// DFT value at x.  Bluestein: X[x] = chirp_x conj(c_x), c = FFT(conj(FFT(a)
+// bhat)) with bhat = FFT(conj chirp) / M; the pointwise steps run inside the
+// first and last Stockham passes of the two FFTs.
+template <int LO, int HI, int TPF = 64, bool PRECHIRP = false>
 __device__ __forceinline__ void wave_ring_dft(float2* buf, const RingPair& pr,
                                               const float2* __restrict__ bhat,
                                               const float2* __restrict__ chirp,
                                               const float2* __restrict__ tw, int log2_tab, int lane) {
-  wave_fft<LO, HI, TPF>(buf, pr.log2M, tw, log2_tab, lane);
-  if (pr.M != pr.nphi) {
-    constexpr int kItC = ((1 << HI) / 2 + TPF - 1) / TPF;
-    constexpr bool kEarly = HI <= 11;
-    float2 cw[kEarly ? kItC : 1];
-    // all loads of the kernel spectrum in flight before the LDS updates
-    constexpr int kIt = ((1 << HI) + TPF - 1) / TPF;
-    float2 bh[kIt];
-#pragma unroll
-    for (int i = 0; i < kIt; ++i) {
-      const int x = lane + TPF * i;
-      if (x < pr.M) bh[i] = bhat[x];
-    }
-#pragma unroll
-    for (int i = 0; i < kIt; ++i) {
-      const int x = lane + TPF * i;
-      const int px = pad(lane) + (TPF + TPF / 16) * i;
-      if (x < pr.M) buf[px] = cconj(cmul(buf[px], bh[i]));
-    }
-    if constexpr (kEarly) {
-#pragma unroll
-      for (int i = 0; i < kItC; ++i) {
-        const int x = lane + TPF * i;
-        if (x < pr.nphi) cw[i] = chirp[x];
-      }
-    }
-    field_sync<TPF>();
+  if (pr.M == pr.nphi) {
     wave_fft<LO, HI, TPF>(buf, pr.log2M, tw, log2_tab, lane);
-#pragma unroll
-    for (int i = 0; i < kItC; ++i) {
-      const int x = lane + TPF * i;
-      const int px = pad(lane) + (TPF + TPF / 16) * i;
-      if (x < pr.nphi) {
-        const float2 w = kEarly ? cw[kEarly ? i : 0] : chirp[x];
-        buf[px] = cmul(w, cconj(buf[px]));
-      }
-    }
-    field_sync<TPF>();
+    return;
   }
-}
-
-// seq[x] *= chirp[x] for x < N (Bluestein pre-multiply of a packed sequence).
-template <int HI, int TPF = 64>
-__device__ __forceinline__ void chirp_premultiply(float2* buf, const float2* __restrict__ chirp, int N,
-                                                  int lane) {
-  constexpr int kItC = ((1 << HI) / 2 + TPF - 1) / TPF;
-  constexpr int kB = kItC < 8 ? kItC : 8;
-#pragma unroll
-  for (int i0 = 0; i0 < kItC; i0 += kB) {
-    float2 w[kB];
-#pragma unroll
-    for (int u = 0; u < kB; ++u) {
-      const int x = lane + TPF * (i0 + u);
-      if (x < N) w[u] = chirp[x];
-    }
-#pragma unroll
-    for (int u = 0; u < kB; ++u) {
-      const int x = lane + TPF * (i0 + u);
-      const int px = pad(lane) + (TPF + TPF / 16) * (i0 + u);
-      if (x < N) buf[px] = cmul(buf[px], w[u]);
-    }
-  }
+  wave_fft<LO, HI, TPF, PRECHIRP ? 1 : 0, 1>(buf, pr.log2M, tw, log2_tab, lane, chirp, bhat, pr.nphi);
+  wave_fft<LO, HI, TPF, 0, 2>(buf, pr.log2M, tw, log2_tab, lane, nullptr, chirp, pr.nphi);
 }
 
 template <bool PADDED>
@@ -604,11 +593,7 @@ __global__ __launch_bounds__(WAVES * TPF) void ring_synthesis_wave_kernel(const 
   // every field runs it (fields beyond the batch on zeros)
   if (TPF == 64 && b >= a.nbatch) return;
   float2* buf = lds + wave * stride;
-  if (M != N) {
-    chirp_premultiply<HI, TPF>(buf, chirp, N, lane);
-    field_sync<TPF>();
-  }
-  wave_ring_dft<LO, HI, TPF>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
+  wave_ring_dft<LO, HI, TPF, true>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
   if (b >= a.nbatch) return;
   float* fr = f + b * a.npix;
   constexpr int kIt = ((1 << HI) + TPF - 1) / TPF;
