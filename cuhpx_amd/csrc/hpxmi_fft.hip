@@ -24,6 +24,9 @@
 // S = F_t + F_t', D = F_t - F_t' (the Legendre step's operands for l+m even /
 // odd) are fused into the load and store loops; the folded spectra are
 // written through LDS so that consecutive lanes write consecutive fields.
+// Bluestein's pointwise steps run inside the first and last Stockham passes,
+// and the first pass of an analysis reads the map (the last pass of a
+// synthesis writes it) directly.
 // Rings that need an 8192-point FFT (nside > 512) use a work-group-wide
 // variant of the same Stockham passes.
 
@@ -178,16 +181,34 @@ __device__ __forceinline__ void field_sync() {
 // One radix-R Stockham pass over M = 2^LOG2M points by one wavefront, after
 // 2^LOG2NS points have been combined.  All strides are compile-time constants,
 // so the LDS accesses use immediate offsets.
+// The global-memory ends of a ring pair's DFT, used by its first and last
+// passes (n: the ring length; 0 for a field beyond the batch, which then
+// reads zeros and writes nothing).
+struct PassIO {
+  const float2* pro = nullptr;  // pre-multiply factors (chirp) or nullptr
+  const float2* epi = nullptr;  // post-multiply factors
+  const float* in_n = nullptr;  // ring samples, north / south ring
+  const float* in_s = nullptr;
+  float* out_n = nullptr;       // output rings
+  float* out_s = nullptr;
+  int n = 0;
+  int single = 0;               // the equator: one real ring
+};
+
 // TPF threads share the sequence (lane = thread index within the field).
-// Pointwise factors fused into the pass (the Bluestein steps):
-//   PRO = 1 (first pass): input x *= pro[x] for x < n (chirp pre-multiply);
-//   EPI = 1 (last pass): output x = conj(output x * epi[x]) (kernel spectrum);
-//   EPI = 2 (last pass): output x = epi[x] conj(output x) for x < n (chirp).
-// In the first pass input x = b + r kNB; in the last pass output x = b + r kNS.
+// Ends fused into the pass (input x = b + r kNB in the first pass, output
+// x = b + r kNS in the last):
+//   PRO = 1: input x *= pro[x] for x < n (Bluestein chirp pre-multiply);
+//   PRO = 2: input x = (in_n[x], in_s[x]) (times pro[x] if given) for x < n,
+//            0 beyond, read from global memory instead of LDS;
+//   EPI = 1: output x = conj(output x * epi[x]) (Bluestein kernel spectrum);
+//   EPI = 2: output x = epi[x] conj(output x) for x < n (chirp post-multiply);
+//   EPI = 3: as 2, written to out_n[x] (real part), out_s[x] (minus the
+//            imaginary part) instead of LDS;
+//   EPI = 4: output x written to out_n / out_s likewise, for x < n.
 template <int R, int LOG2M, int LOG2NS, int TPF = 64, int PRO = 0, int EPI = 0>
 __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict__ tw, int log2_tab,
-                                          int lane, const float2* __restrict__ pro = nullptr,
-                                          const float2* __restrict__ epi = nullptr, int n = 0) {
+                                          int lane, const PassIO& io = PassIO{}) {
   constexpr int kLogR = log2_of<R>();
   constexpr int kNB = 1 << (LOG2M - kLogR);  // butterflies; also the input stride
   constexpr int kNS = 1 << LOG2NS;
@@ -209,39 +230,64 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
   for (int i = 0; i < kB; ++i) {
     const int b = lane + TPF * i;
     if (kNB >= TPF || b < kNB) {
-      const int pb = pad(b);
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        if constexpr (kNB % 16 == 0)
-          v[i][r] = buf[pb + r * kNB + ((r * kNB) >> 4)];
-        else
-          v[i][r] = buf[pad(b + r * kNB)];
-      }
-    }
-  }
-  // the pointwise factors, requested while the LDS reads complete
-  float2 f[EPI ? kB : (PRO ? kB : 1)][(EPI || PRO) ? R : 1];
-  if constexpr (PRO || EPI) {
-#pragma unroll
-    for (int i = 0; i < kB; ++i) {
-      const int b = lane + TPF * i;
-      if (kNB >= TPF || b < kNB) {
+      if constexpr (PRO == 2) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int x = PRO ? b + r * kNB : b + r * kNS;
-          if (EPI == 1 || x < n) f[i][r] = PRO ? pro[x] : epi[x];
+          const int x = b + r * kNB;
+          v[i][r] = make_float2(0.f, 0.f);
+          if (x < io.n) v[i][r] = make_float2(io.in_n[x], io.single ? 0.f : io.in_s[x]);
+        }
+      } else {
+        const int pb = pad(b);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if constexpr (kNB % 16 == 0)
+            v[i][r] = buf[pb + r * kNB + ((r * kNB) >> 4)];
+          else
+            v[i][r] = buf[pad(b + r * kNB)];
         }
       }
     }
   }
-  if constexpr (PRO) {
+  // the pointwise factors, requested while the reads complete
+  constexpr bool kPreMul = PRO == 1 || PRO == 2;
+  constexpr bool kPostMul = EPI == 1 || EPI == 2 || EPI == 3;
+  float2 fpre[kPreMul ? kB : 1][kPreMul ? R : 1];
+  float2 f[kPostMul ? kB : 1][kPostMul ? R : 1];
+  if constexpr (kPreMul) {
+    if (io.pro != nullptr) {
+#pragma unroll
+      for (int i = 0; i < kB; ++i) {
+        const int b = lane + TPF * i;
+        if (kNB >= TPF || b < kNB) {
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (b + r * kNB < io.n) fpre[i][r] = io.pro[b + r * kNB];
+        }
+      }
+    }
+  }
+  if constexpr (kPostMul) {
 #pragma unroll
     for (int i = 0; i < kB; ++i) {
       const int b = lane + TPF * i;
       if (kNB >= TPF || b < kNB) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          if (b + r * kNB < n) v[i][r] = cmul(v[i][r], f[i][r]);
+          if (EPI == 1 || b + r * kNS < io.n) f[i][r] = io.epi[b + r * kNS];
+      }
+    }
+  }
+  if constexpr (kPreMul) {
+    if (io.pro != nullptr) {
+#pragma unroll
+      for (int i = 0; i < kB; ++i) {
+        const int b = lane + TPF * i;
+        if (kNB >= TPF || b < kNB) {
+#pragma unroll
+          for (int r = 0; r < R; ++r)
+            if (b + r * kNB < io.n) v[i][r] = cmul(v[i][r], fpre[i][r]);
+        }
       }
     }
   }
@@ -256,14 +302,25 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
       if constexpr (EPI == 1) {
 #pragma unroll
         for (int r = 0; r < R; ++r) v[i][r] = cconj(cmul(v[i][r], f[i][r]));
-      } else if constexpr (EPI == 2) {
+      } else if constexpr (EPI == 2 || EPI == 3) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          if (b + r * kNS < n) v[i][r] = cmul(f[i][r], cconj(v[i][r]));
+          if (b + r * kNS < io.n) v[i][r] = cmul(f[i][r], cconj(v[i][r]));
       }
-      const int pbase = pad((b - jn) * R + jn);
+      if constexpr (EPI == 3 || EPI == 4) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[pbase + r * kNS + ((r * kNS) >> 4)] = v[i][r];
+        for (int r = 0; r < R; ++r) {
+          const int x = b + r * kNS;
+          if (x < io.n) {
+            io.out_n[x] = v[i][r].x;
+            if (!io.single) io.out_s[x] = -v[i][r].y;
+          }
+        }
+      } else {
+        const int pbase = pad((b - jn) * R + jn);
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[pbase + r * kNS + ((r * kNS) >> 4)] = v[i][r];
+      }
     }
   }
   field_sync<TPF>();
@@ -274,14 +331,13 @@ __device__ __forceinline__ void wave_pass(float2* buf, const float2* __restrict_
 // passes, 4096 in 3).
 template <int LOG2M, int LOG2NS = 0, int TPF = 64, int PRO = 0, int EPI = 0>
 __device__ __forceinline__ void wave_fft_fixed(float2* buf, const float2* __restrict__ tw,
-                                               int log2_tab, int lane, const float2* __restrict__ pro = nullptr,
-                                               const float2* __restrict__ epi = nullptr, int n = 0) {
+                                               int log2_tab, int lane, const PassIO& io = PassIO{}) {
   if constexpr (LOG2NS < LOG2M) {
     constexpr int rem = LOG2M - LOG2NS;
     constexpr int R = (rem >= 6 || rem == 4) ? 16 : ((rem == 3 || rem == 5) ? 8 : (rem == 2 ? 4 : 2));
     constexpr bool first = LOG2NS == 0, last = LOG2NS + log2_of<R>() == LOG2M;
-    wave_pass<R, LOG2M, LOG2NS, TPF, first ? PRO : 0, last ? EPI : 0>(buf, tw, log2_tab, lane, pro, epi, n);
-    wave_fft_fixed<LOG2M, LOG2NS + log2_of<R>(), TPF, PRO, EPI>(buf, tw, log2_tab, lane, pro, epi, n);
+    wave_pass<R, LOG2M, LOG2NS, TPF, first ? PRO : 0, last ? EPI : 0>(buf, tw, log2_tab, lane, io);
+    wave_fft_fixed<LOG2M, LOG2NS + log2_of<R>(), TPF, PRO, EPI>(buf, tw, log2_tab, lane, io);
   }
 }
 
@@ -289,34 +345,39 @@ __device__ __forceinline__ void wave_fft_fixed(float2* buf, const float2* __rest
 // for the sizes of one size class (log2M in [LO, HI]).
 template <int LO, int HI, int TPF = 64, int PRO = 0, int EPI = 0>
 __device__ __forceinline__ void wave_fft(float2* buf, int log2M, const float2* __restrict__ tw,
-                                         int log2_tab, int lane, const float2* __restrict__ pro = nullptr,
-                                         const float2* __restrict__ epi = nullptr, int n = 0) {
+                                         int log2_tab, int lane, const PassIO& io = PassIO{}) {
   if constexpr (LO <= HI) {
     if (log2M == LO) {
-      wave_fft_fixed<LO, 0, TPF, PRO, EPI>(buf, tw, log2_tab, lane, pro, epi, n);
+      wave_fft_fixed<LO, 0, TPF, PRO, EPI>(buf, tw, log2_tab, lane, io);
       return;
     }
-    wave_fft<LO + 1, HI, TPF, PRO, EPI>(buf, log2M, tw, log2_tab, lane, pro, epi, n);
+    wave_fft<LO + 1, HI, TPF, PRO, EPI>(buf, log2M, tw, log2_tab, lane, io);
   }
 }
 
-// DFT of length N of the wave's sequence (entered as z_j, or for Bluestein
-// z_j * chirp_j zero-padded to M -- PRECHIRP: z_j, multiplied by chirp_j in
-// the first pass), in place: afterwards point x < N of the sequence holds the
-// DFT value at x.  Bluestein: X[x] = chirp_x conj(c_x), c = FFT(conj(FFT(a)
-// bhat)) with bhat = FFT(conj chirp) / M; the pointwise steps run inside the
-// first and last Stockham passes of the two FFTs.
-template <int LO, int HI, int TPF = 64, bool PRECHIRP = false>
+// DFT of length N of a ring pair's packed sequence z_j = f_n[j] + i f_s[j]
+// (zero-padded to M for Bluestein, with X[x] = chirp_x conj(c_x),
+// c = FFT(conj(FFT(z chirp) bhat)), bhat = FFT(conj chirp) / M; the pointwise
+// steps run inside the first and last passes of the two FFTs).
+//   ANALYSIS: z is read from io.in_n / io.in_s; afterwards point x < N of the
+//     wave's LDS sequence holds the DFT value at x.
+//   synthesis: z is the LDS sequence; the real and minus the imaginary parts
+//     of the DFT values x < N go to io.out_n / io.out_s.
+template <int LO, int HI, int TPF, bool ANALYSIS>
 __device__ __forceinline__ void wave_ring_dft(float2* buf, const RingPair& pr,
                                               const float2* __restrict__ bhat,
                                               const float2* __restrict__ chirp,
-                                              const float2* __restrict__ tw, int log2_tab, int lane) {
+                                              const float2* __restrict__ tw, int log2_tab, int lane, PassIO io) {
   if (pr.M == pr.nphi) {
-    wave_fft<LO, HI, TPF>(buf, pr.log2M, tw, log2_tab, lane);
+    io.pro = nullptr;
+    wave_fft<LO, HI, TPF, ANALYSIS ? 2 : 0, ANALYSIS ? 0 : 4>(buf, pr.log2M, tw, log2_tab, lane, io);
     return;
   }
-  wave_fft<LO, HI, TPF, PRECHIRP ? 1 : 0, 1>(buf, pr.log2M, tw, log2_tab, lane, chirp, bhat, pr.nphi);
-  wave_fft<LO, HI, TPF, 0, 2>(buf, pr.log2M, tw, log2_tab, lane, nullptr, chirp, pr.nphi);
+  io.pro = chirp;
+  io.epi = bhat;
+  wave_fft<LO, HI, TPF, ANALYSIS ? 2 : 1, 1>(buf, pr.log2M, tw, log2_tab, lane, io);
+  io.epi = chirp;
+  wave_fft<LO, HI, TPF, 0, ANALYSIS ? 2 : 3>(buf, pr.log2M, tw, log2_tab, lane, io);
 }
 
 template <bool PADDED>
@@ -456,7 +517,6 @@ __global__ __launch_bounds__(WAVES * TPF) void ring_analysis_wave_kernel(const f
   const int64_t b0 = (item % nfg) * WAVES;
   const RingPair pr = a.pairs[k];
   const int N = pr.nphi, M = pr.M;
-  const bool blue = M != N;
   const float2* chirp = a.chirp + pr.chirp_off;
   const int wave = threadIdx.x / TPF, lane = threadIdx.x % TPF;
   const int64_t b = b0 + wave;
@@ -469,40 +529,16 @@ __global__ __launch_bounds__(WAVES * TPF) void ring_analysis_wave_kernel(const f
     for (int m = threadIdx.x; m < a.mmax; m += kThreadsW) phs[m] = ph[m];
   }
   // with several wavefronts per field the DFT has work-group barriers, so
-  // every field runs it (fields beyond the batch on zeros)
+  // every field runs it (fields beyond the batch on zeros); the first pass
+  // reads the ring samples straight from the map
   const bool active = b < a.nbatch;
   if (active || TPF > 64) {
-    const float* fn = f + (active ? b : 0) * a.npix + pr.off_n;
-    const float* fs = f + (active ? b : 0) * a.npix + (pr.single ? pr.off_n : pr.off_s);
-    constexpr int kIt = ((1 << HI) + TPF - 1) / TPF;
-    // all ring samples of the field in flight at once (one round trip)
-    constexpr int kBatch = kIt < 32 ? kIt : 32;
-#pragma unroll
-    for (int i0 = 0; i0 < kIt; i0 += kBatch) {
-      float2 z[kBatch], c[kBatch];
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int j = lane + TPF * (i0 + u);
-        z[u] = make_float2(0.f, 0.f);
-        if (active && j < N) {
-          z[u].x = fn[j];
-          z[u].y = fs[j];
-          if (blue) c[u] = chirp[j];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < kBatch; ++u) {
-        const int j = lane + TPF * (i0 + u);
-        if (j < M) {
-          float2 v = z[u];
-          if (pr.single) v.y = 0.f;
-          if (blue && j < N) v = cmul(v, c[u]);
-          buf[pad(lane) + (TPF + TPF / 16) * (i0 + u)] = v;
-        }
-      }
-    }
-    field_sync<TPF>();
-    wave_ring_dft<LO, HI, TPF>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
+    PassIO io;
+    io.in_n = f + (active ? b : 0) * a.npix + pr.off_n;
+    io.in_s = f + (active ? b : 0) * a.npix + (pr.single ? pr.off_n : pr.off_s);
+    io.n = active ? N : 0;
+    io.single = pr.single;
+    wave_ring_dft<LO, HI, TPF, true>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane, io);
   }
   __syncthreads();
   analysis_store<WAVES, TPF>(a, pr, k, b0, b, lds, stride, phs, X, lane);
@@ -593,19 +629,13 @@ __global__ __launch_bounds__(WAVES * TPF) void ring_synthesis_wave_kernel(const 
   // every field runs it (fields beyond the batch on zeros)
   if (TPF == 64 && b >= a.nbatch) return;
   float2* buf = lds + wave * stride;
-  wave_ring_dft<LO, HI, TPF, true>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane);
-  if (b >= a.nbatch) return;
-  float* fr = f + b * a.npix;
-  constexpr int kIt = ((1 << HI) + TPF - 1) / TPF;
-#pragma unroll
-  for (int u = 0; u < kIt; ++u) {
-    const int j = lane + TPF * u;
-    if (j < N) {
-      const float2 r = buf[pad(lane) + (TPF + TPF / 16) * u];
-      fr[pr.off_n + j] = r.x;
-      if (!pr.single) fr[pr.off_s + j] = -r.y;
-    }
-  }
+  // the last pass writes the two rings of the map
+  PassIO io;
+  io.out_n = f + (b < a.nbatch ? b : 0) * a.npix + pr.off_n;
+  io.out_s = f + (b < a.nbatch ? b : 0) * a.npix + pr.off_s;
+  io.n = b < a.nbatch ? N : 0;
+  io.single = pr.single;
+  wave_ring_dft<LO, HI, TPF, false>(buf, pr, a.bhat + pr.bhat_off, chirp, a.tw, a.log2_tab, lane, io);
 }
 
 // ---------------------------------------------------------------------------
