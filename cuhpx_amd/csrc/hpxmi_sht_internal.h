@@ -14,14 +14,14 @@ namespace hpxmi {
 // wavefront) per work-group for the wave-level size classes 0..2, for the
 // analysis [0] and synthesis [1] directions.
 struct FftTune {
-  int waves[2][3] = {{16, 8, 2}, {16, 8, 2}};       // fields per work-group [direction][class]
+  int waves[2][3] = {{16, 4, 2}, {16, 4, 2}};       // fields per work-group [direction][class]
   int tpf[2][3] = {{64, 128, 64}, {64, 128, 64}};   // threads per field (64, 128 or 256)
 };
 // Launch shapes of the Legendre GEMMs for one (precision, column count):
 // 16-column MFMA tiles per wavefront (work-group column block = 64 * ct).
 struct GemmTune {
-  int fwd_ct = 2;
-  int inv_ct = 2;
+  int fwd_ct = 1;  // 1: what the tuner picks on MI355X at 64 fields (4 waves per SIMD)
+  int inv_ct = 1;
   int inv_rh = 1;  // inverse: 64-ring-pair row halves per work-group (1 or 2)
 };
 // Allowed values (what the kernels are instantiated for).
@@ -100,9 +100,7 @@ struct ShtPlan {
     std::lock_guard<std::mutex> g(tune_mu);
     auto it = gemm_tunes.find({precision, ncol});
     if (it != gemm_tunes.end()) return it->second;
-    GemmTune t;
-    if (ncol <= 64) t.fwd_ct = t.inv_ct = 1;  // one column block keeps all waves busy
-    return t;
+    return GemmTune{};
   }
 };
 
