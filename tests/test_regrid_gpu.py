@@ -17,7 +17,7 @@ def rel_l2(x, ref):
     return float(np.linalg.norm(np.asarray(x) - ref) / np.linalg.norm(ref))
 
 
-@pytest.mark.parametrize("grid", ["equiangular", "legendre-gauss"])
+@pytest.mark.parametrize("grid", ["equiangular", "legendre-gauss", "lobatto"])
 @pytest.mark.parametrize("csphase", [True, False])
 def test_latlon_legs_match_oracle(grid, csphase):
     nlat, nlon, L = 16, 32, 15
