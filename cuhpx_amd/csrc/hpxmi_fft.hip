@@ -12,13 +12,15 @@
 // MI355X design.  The two real rings of a mirror pair (north t and south
 // nring-1-t share length, phase and weight) are packed into one complex
 // sequence z = f_t + i f_t', so one complex DFT serves both.  Each DFT is
-// owned by one wavefront and runs in that wave's slice of LDS: a radix-8/4
-// Stockham FFT whose passes need no work-group barrier (LDS accesses of one
-// wavefront complete in program order), so the waves of a work-group progress
-// independently.  Rings whose length is not a power of two use Bluestein's
-// chirp-z algorithm with a padded power-of-two length M >= 2N-1 and FP64-built
-// chirp / kernel-spectrum tables.  A work-group holds one ring pair and one
-// wave per field; pairs are launched in classes of equal maximum FFT length so
+// owned by 1, 2 or 4 wavefronts (a launch shape the autotuner picks) and runs
+// in the field's slice of LDS: a radix-16/8/4/2 Stockham FFT.  With one
+// wavefront per field the passes need no work-group barrier (LDS accesses of
+// one wavefront complete in program order), so the waves of a work-group
+// progress independently; wider fields synchronise the work-group between
+// passes.  Rings whose length is not a power of two use Bluestein's chirp-z
+// algorithm with a padded power-of-two length M >= 2N-1 and FP64-built
+// chirp / kernel-spectrum tables.  A work-group holds one ring pair and a
+// few fields; pairs are launched in classes of equal maximum FFT length so
 // that the LDS allocation fits the class.  Chirp multiply, phase shift,
 // quadrature weight, the split into the two rings and the equatorial folding
 // S = F_t + F_t', D = F_t - F_t' (the Legendre step's operands for l+m even /

@@ -19,14 +19,15 @@
 //    v_mfma_f32_16x16x32_bf16 for the BF16 paths.  The 3-term split product
 //    hi*hi + hi*lo + lo*hi (x = hi + lo, both BF16, split from the FP32
 //    operands; the lo*lo term is below FP32 rounding) accumulates in FP32.
-//  * Forward work-group: one m, 64 rows of each parity, 128 columns
-//    (64 fields, re/im).  The table rows are staged once through LDS and
-//    shared by 4 waves; each wave streams its 32 columns of the spectra
-//    straight into MFMA B registers.  The output a[b][l][m] is m-minor, so the
-//    work list puts consecutive m of the same row block on the same XCD,
-//    whose L2 merges the partial-line writes.
-//  * Inverse work-group: one m, 64 ring pairs, 128 columns; the table tile is
-//    transposed through LDS.
+//  * Forward work-group: one m, 64 rows of each parity, 64 or 128 columns
+//    (re/im of 32 or 64 fields; 16 or 32 per wave).  The table rows are
+//    staged once per k chunk through LDS and shared by the 4 waves; each wave
+//    streams its columns of the spectra straight into MFMA B registers.  The
+//    output aT[m][parity][i][col] leaves through LDS as whole row segments;
+//    coeff_to_user_kernel turns it into the user layout a[b][l][m].
+//  * Inverse work-group: one m, 64 (or 128) ring pairs, 64 or 128 columns;
+//    A tiles come from the transposed table PT[m][parity][k][i], built once
+//    at plan time, so they stage into LDS without a transposition.
 
 #include "hpxmi_sht_internal.h"
 #include "../../include/hpxmi.h"

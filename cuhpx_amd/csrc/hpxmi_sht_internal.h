@@ -10,9 +10,9 @@
 
 namespace hpxmi {
 
-// Launch shapes of the ring FFT stage for one column count: fields (one per
-// wavefront) per work-group for the wave-level size classes 0..2, for the
-// analysis [0] and synthesis [1] directions.
+// Launch shapes of the ring FFT stage for one column count: fields per
+// work-group and threads per field for the wave-level size classes 0..2, for
+// the analysis [0] and synthesis [1] directions.
 struct FftTune {
   int waves[2][3] = {{16, 4, 2}, {16, 4, 2}};       // fields per work-group [direction][class]
   int tpf[2][3] = {{64, 128, 64}, {64, 128, 64}};   // threads per field (64, 128 or 256)
