@@ -108,12 +108,10 @@ struct ShtPlan {
 constexpr int kRowPad = 16;
 // Ring-pair (K) tile of the Legendre GEMMs.
 constexpr int kKTile = 32;
-// Fields per FFT work-group.
-constexpr int kFieldsPerFftBlock = 8;
 // Largest FFT length (complex points) of the ring FFT kernels.
 constexpr int kFftLds = 8192;
-// FFT size classes: log2 M <= 10, 11, 12 (one wavefront per transform) and
-// 13 (one work-group per transform).
+// FFT size classes: log2 M <= 10, 11, 12 (1, 2 or 4 wavefronts per
+// transform) and 13 (one work-group per transform).
 constexpr int kFftClasses = 4;
 int fft_class_of(int log2M);
 
