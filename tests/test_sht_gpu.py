@@ -273,3 +273,27 @@ def test_unsupported_nside_is_a_clean_error():
     """nside 2048 needs 16384-point Bluestein FFTs: plan creation refuses it."""
     with pytest.raises(NotImplementedError):
         ShtPlan.healpix(2048, 9, 9)
+
+
+def test_empty_batches_pass_through():
+    """Zero maps in, zero maps out, with the leading dims kept; no kernel runs."""
+    nside = 8
+    npix = 12 * nside * nside
+    sht, isht = cuhpx_amd.SHTCUDA(nside), cuhpx_amd.iSHTCUDA(nside)
+    x = torch.zeros((0, npix), device="cuda", requires_grad=True)
+    a = sht(x)
+    assert a.shape == (0, sht.lmax, sht.mmax) and a.dtype == torch.complex64
+    a.abs().sum().backward()
+    assert x.grad.shape == (0, npix)
+    f = isht(a.detach())
+    assert f.shape == (0, npix) and f.dtype == torch.float32
+    assert sht(torch.zeros((2, 0, npix), device="cuda")).shape == (2, 0, sht.lmax, sht.mmax)
+    vf, vi = cuhpx_amd.VectorSHT(nside), cuhpx_amd.VectoriSHT(nside)
+    c = vf(torch.zeros((0, 2, npix), device="cuda"))
+    assert c.shape == (0, 2, vf.lmax, vf.mmax)
+    assert vi(c).shape == (0, 2, npix)
+    for dtype in (torch.float32, torch.int64, torch.uint8):
+        e = torch.zeros((0, npix), dtype=dtype, device="cuda")
+        assert cuhpx_amd.nest2ring(e, nside).shape == (0, npix)
+        assert cuhpx_amd.flat2flat(e, "S", False, "N", True, nside).shape == (0, npix)
+    torch.cuda.synchronize()
