@@ -142,3 +142,31 @@ def test_full_size_nside1024_batch64(kind, args):
     assert torch.equal(oi, inv_t.expand(rows, npix))
     of = _call(kind, xf, nside, *args)
     assert torch.equal(of, torch.gather(xf, 1, oi.long()))
+
+
+def test_full_size_nside1024_all_origins():
+    """Origin coverage at the measured size ([64, 12*1024^2] int32 index rows):
+    nest2flat for all four origins in both directions against the oracle
+    permutation, then flat2flat for all 64 (source, destination) pairs, whose
+    output must be the destination's flat layout of the same map:
+    flat2flat(a -> b)(nest2flat_a(x)) = nest2flat_b(x)."""
+    nside, rows = 1024, 64
+    npix = 12 * nside * nside
+    xi = torch.arange(npix, dtype=torch.int32, device="cuda").expand(rows, npix).contiguous()
+    combos = [(o, cw) for o in "ESWN" for cw in (False, True)]
+    flats = {}
+    for o, cw in combos:
+        print(f"[full-size] nest2flat {o} cw={cw}", flush=True)
+        sig = R.sigma("nest2flat", nside, o, cw)
+        inv = np.empty_like(sig)
+        inv[sig] = np.arange(npix)
+        out = cuhpx_amd.nest2flat(xi, o, cw, nside)
+        assert torch.equal(out, torch.from_numpy(inv.astype(np.int32)).cuda().expand(rows, npix))
+        flats[(o, cw)] = out
+    print("[full-size] flat2flat, 64 pairs", flush=True)
+    for (so, scw) in combos:
+        for (do, dcw) in combos:
+            got = cuhpx_amd.flat2flat(flats[(so, scw)], so, scw, do, dcw, nside)
+            assert torch.equal(got, flats[(do, dcw)]), (so, scw, do, dcw)
+            del got
+    torch.cuda.synchronize()
