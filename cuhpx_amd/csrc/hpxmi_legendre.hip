@@ -430,7 +430,7 @@ __device__ __forceinline__ float vec_sign1(int lane, bool forward) {
 template <int PREC, int CT, bool VEC = false>
 __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_fwd_kernel(
     const float* __restrict__ X, float* __restrict__ aT, const float* __restrict__ P,
-    const int64_t* __restrict__ p_off, const int* __restrict__ nl,
+    const int64_t* __restrict__ p_off, const int* __restrict__ nl, const int* __restrict__ kmin,
     const int2* __restrict__ items, int nitems, int ncb, int kpad, int64_t ncol, int64_t vstride) {
   // table tiles (double buffer) during the main loop, the output tile after it
   constexpr size_t kTileBytes = sizeof(float) * 64 * (64 * CT + 4);
@@ -534,27 +534,36 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
     }
   };
 
+  // k chunks wholly below kmin[m] hold zero spectra and are skipped (in
+  // both halves of a vector reduction); the others run in the same order,
+  // so every sum is unchanged.  Chunk t of the loop starts at ring pair kof(t).
+  const int c0 = min(kmin[m], kpad) / kKTile;
+  const int half = kpad / kKTile - c0;
+  const int nchunk = (VEC ? 2 : 1) * half;
+  auto kof = [&](int t) { return (VEC && t >= half ? kpad + (t - half) * kKTile : t * kKTile) + c0 * kKTile; };
+
   // software pipeline over k chunks with two register sets for B and two
   // LDS buffers for the table tile (one barrier per chunk, no copies)
-  const int nchunk = (VEC ? 2 : 1) * kpad / kKTile;
   float pa[16];
   float b0[2][CT][8], b1[2][CT][8];
-  load_a(0, pa);
-  if (wave_active) load_b(0, b0);
-  store_a(lds[0], pa);
-  __syncthreads();
+  if (nchunk > 0) {
+    load_a(kof(0), pa);
+    if (wave_active) load_b(kof(0), b0);
+    store_a(lds[0], pa);
+    __syncthreads();
+  }
   for (int c = 0; c < nchunk; c += 2) {
     if (c + 1 < nchunk) {
-      load_a((c + 1) * kKTile, pa);
-      if (wave_active) load_b((c + 1) * kKTile, b1);
+      load_a(kof(c + 1), pa);
+      if (wave_active) load_b(kof(c + 1), b1);
     }
     if (wave_active) mfma_chunk<PREC, CT>(lds[0], b0, acc, lane);
     if (c + 1 < nchunk) store_a(lds[1], pa);
     __syncthreads();
     if (c + 1 >= nchunk) break;
     if (c + 2 < nchunk) {
-      load_a((c + 2) * kKTile, pa);
-      if (wave_active) load_b((c + 2) * kKTile, b0);
+      load_a(kof(c + 2), pa);
+      if (wave_active) load_b(kof(c + 2), b0);
     }
     if (wave_active) mfma_chunk<PREC, CT>(lds[1], b1, acc, lane);
     if (c + 2 < nchunk) store_a(lds[0], pa);
@@ -584,8 +593,8 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
 template <int PREC, int CT, int RH, bool VEC = false>
 __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_kernel(
     const float* __restrict__ aT, float* __restrict__ X, const float* __restrict__ PT,
-    const int64_t* __restrict__ p_off, const int* __restrict__ nl, int nkb, int nitems, int ncb, int kpad,
-    int64_t ncol, int64_t at_rows, int64_t vstride) {
+    const int64_t* __restrict__ p_off, const int* __restrict__ nl, const int* __restrict__ kmin, int nkb,
+    int nitems, int ncb, int kpad, int64_t ncol, int64_t at_rows, int64_t vstride) {
   // same tiling as the forward kernel with the roles of i and k exchanged:
   // rows = ring pairs, reduction over the rows i of aT, A tiles from the
   // transposed table PT[m][p][k][i]
@@ -617,6 +626,9 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
   const int padp[2] = {(nlp[0] + kRowPad - 1) / kRowPad * kRowPad, (nlp[1] + kRowPad - 1) / kRowPad * kRowPad};
   const int nmax = nlp[0] > nlp[1] ? nlp[0] : nlp[1];
   const int k0 = kb * kRows;
+  // ring pairs below kmin[m] are read by no synthesis at this m (their
+  // spectra stop below m): a block wholly below it has nothing to compute
+  if (k0 + kRows <= kmin[m]) return;
   const int nc = (int)ncol;
 
   floatx4 acc[2][4][CT];
@@ -946,11 +958,13 @@ int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, flo
 #define HPX_FWD(PR, CT)                                                                                       \
   do {                                                                                                        \
     if (p.vector)                                                                                             \
-      legendre_fwd_kernel<PR, CT, true><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items, \
-                                                                  p.n_fwd_items, (int)ncb, p.kpad, ncol, p.vstride); \
+      legendre_fwd_kernel<PR, CT, true><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, p.d_kmin,     \
+                                                                  p.d_fwd_items, p.n_fwd_items, (int)ncb, p.kpad, \
+                                                                  ncol, p.vstride);                             \
     else                                                                                                      \
-      legendre_fwd_kernel<PR, CT><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, p.d_fwd_items,  \
-                                                            p.n_fwd_items, (int)ncb, p.kpad, ncol, 0);       \
+      legendre_fwd_kernel<PR, CT><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, p.d_kmin,        \
+                                                            p.d_fwd_items, p.n_fwd_items, (int)ncb, p.kpad,   \
+                                                            ncol, 0);                                         \
   } while (0)
   switch (precision * 4 + ct) {
     case HPX_PREC_FP32 * 4 + 1: HPX_FWD(HPX_PREC_FP32, 1); break;
@@ -980,11 +994,11 @@ int launch_gemm_inv(const ShtPlan& p, int precision, int ct, int rh, const float
 #define HPX_INV(PR, CT, RH)                                                                                    \
   do {                                                                                                         \
     if (RH == 1 && p.vector)                                                                                   \
-      legendre_inv_kernel<PR, CT, 1, true><<<grid, kThreads, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, nkb,    \
+      legendre_inv_kernel<PR, CT, 1, true><<<grid, kThreads, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, p.d_kmin, nkb, \
                                                                      nitems, (int)ncb, p.kpad, ncol, p.at_rows, \
                                                                      p.vstride);                                \
     else                                                                                                       \
-      legendre_inv_kernel<PR, CT, RH><<<grid, kThreads * RH, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, nkb,     \
+      legendre_inv_kernel<PR, CT, RH><<<grid, kThreads * RH, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, p.d_kmin, nkb, \
                                                                      nitems, (int)ncb, p.kpad, ncol, p.at_rows, 0); \
   } while (0)
   switch (precision * 8 + ct * 2 + rh - 1) {

@@ -93,6 +93,7 @@ void free_plan(ShtPlan* p) {
   hipFree(p->d_p_off);
   hipFree(p->d_nl);
   hipFree(p->d_fwd_items);
+  hipFree(p->d_kmin);
   hipFree(p->d_plist);
   delete p;
 }
@@ -216,15 +217,29 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     max_blocks = std::max(max_blocks, (nmax + 63) / 64);
   }
   // forward work list: m outer, row block inner.  The work-groups of one m
-  // read the same spectra X[m] (512 KB at 64 fields) and consecutive m share
-  // output cache lines; the launch maps contiguous ranges of this list to one
-  // XCD, so both stay in that XCD's L2.
-  for (int m = 0; m < mmax; ++m) {
+  // read the same spectra X[m] (512 KB at 64 fields); the launch maps
+  // contiguous ranges of this list to one XCD, so they share that XCD's L2.
+  // Both the row-block count and (with the polar ring pairs skipped, see
+  // kmin below) the reduction length fall with m, so m runs in the order 0,
+  // mmax-1, 1, mmax-2, ...: every contiguous eighth then carries about the
+  // same work.
+  for (int mi = 0; mi < mmax; ++mi) {
+    const int m = (mi & 1) ? mmax - 1 - (mi >> 1) : (mi >> 1);
     const int nmax = std::max(p->nl[2 * m], p->nl[2 * m + 1]);
     for (int jb = 0; jb * 64 < nmax; ++jb) items.push_back(make_int2(m, jb));
   }
   (void)max_blocks;
   p->n_fwd_items = (int)items.size();
+  // kmin[m]: the first ring pair k with K_k = min(mmax, nphi_k / 2 + 1) > m.
+  // The analysis writes zero spectra for m >= K_k and the synthesis reads
+  // only m < K_k, so the forward GEMM skips the k chunks wholly below
+  // kmin[m] and the inverse GEMM the ring-pair blocks wholly below it (the
+  // polar rings at high m: a quarter of the spectra at nside 256, L 513).
+  std::vector<int> kmin(mmax, p->npair);
+  for (int k = p->npair - 1; k >= 0; --k) {
+    const int K = std::min(mmax, p->pairs[k].nphi / 2 + 1);
+    for (int m = 0; m < K; ++m) kmin[m] = k;
+  }
 
   std::vector<double> ct(p->kpad, 0.0);
   for (int k = 0; k < p->npair; ++k) ct[k] = cos_theta[k];
@@ -239,6 +254,7 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
       (rc = upload(&p->d_p_off, p->p_off, bytes)) != HPX_OK ||
       (rc = upload(&p->d_nl, p->nl, bytes)) != HPX_OK ||
       (rc = upload(&p->d_fwd_items, items, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_kmin, kmin, bytes)) != HPX_OK ||
       (rc = upload(&p->d_plist, plist, bytes)) != HPX_OK) {
     free_plan(p);
     return rc;

@@ -73,6 +73,7 @@ struct ShtPlan {
   int64_t* d_p_off = nullptr;   // [mmax + 1]
   int* d_nl = nullptr;          // [mmax * 2]
   int2* d_fwd_items = nullptr;  // forward GEMM work items (m, row block)
+  int* d_kmin = nullptr;        // [mmax] first ring pair whose spectrum reaches order m
   int* d_plist = nullptr;       // ring pairs grouped by FFT size class
   int class_off[4] = {0, 0, 0, 0};
   int class_n[4] = {0, 0, 0, 0};

@@ -224,9 +224,14 @@ int hpx_ring_irfft(const hpx_sht_plan* plan, const float* ftm, float* f, int64_t
  * of the workspace for the coefficients in GEMM order.  hpx_sht_forward is
  * hpx_sht_stage_ring_analysis followed by hpx_sht_stage_legendre_analysis;
  * hpx_sht_inverse is hpx_sht_stage_legendre_synthesis followed by
- * hpx_sht_stage_ring_synthesis.  No counterpart in the reference (its stages
- * are the cuhpx_fft calls and the einsums inside one autograd Function);
- * they let callers time, overlap or pipeline the stages. */
+ * hpx_sht_stage_ring_synthesis.  A ring pair's spectrum stops at
+ * K = min(mmax, nphi / 2 + 1): the ring analysis writes zeros for m >= K,
+ * which the Legendre analysis relies on (it skips the ring pairs whose
+ * spectrum stops below m), and the Legendre synthesis may leave entries
+ * with m >= K unwritten, which the ring synthesis never reads.  No
+ * counterpart in the reference (its stages are the cuhpx_fft calls and the
+ * einsums inside one autograd Function); they let callers time, overlap or
+ * pipeline the stages. */
 int hpx_sht_stage_ring_analysis(const hpx_sht_plan* plan, const float* f, int64_t nbatch,
                                 void* workspace, int64_t workspace_bytes, hipStream_t stream);
 int hpx_sht_stage_legendre_analysis(const hpx_sht_plan* plan, int precision, float* alm,
