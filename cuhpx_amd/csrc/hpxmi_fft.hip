@@ -466,7 +466,7 @@ __device__ __forceinline__ void analysis_store(const StageArgs& a, const RingPai
     // folded spectra X[m][parity][k][2b + re/im]: fields fastest so that
     // consecutive lanes write consecutive columns
     const int64_t pstride = (int64_t)a.kpad * a.ncol;
-    const int total = a.mmax * WAVES;
+    const int total = pr.mwrite * WAVES;  // no further order of this pair is read
     for (int e = threadIdx.x; e < total; e += kThreadsW) {
       const int m = e / WAVES;
       const int g = e - m * WAVES;

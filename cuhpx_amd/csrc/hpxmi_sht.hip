@@ -147,7 +147,7 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     r.off_n = offset[k];
     r.off_s = offset[nring - 1 - k];
     r.weight = (float)weights[k];
-    r.pad_ = 0.f;
+    r.mwrite = mmax;
     r.chirp_off = 0;
     r.bhat_off = 0;
     max_M = std::max(max_M, r.M);
@@ -239,6 +239,18 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   for (int k = p->npair - 1; k >= 0; --k) {
     const int K = std::min(mmax, p->pairs[k].nphi / 2 + 1);
     for (int m = 0; m < K; ++m) kmin[m] = k;
+  }
+  // The forward GEMM reads ring pair k at order m when k's chunk of kKTile
+  // pairs is not below kmin[m]'s; the analysis writes pair k only up to the
+  // last such m (its zeros beyond K_k included), no further.
+  {
+    std::vector<int> mread(p->npair, 0);
+    for (int m = 0; m < mmax; ++m)
+      for (int k = kmin[m] / kKTile * kKTile; k < p->npair; ++k) mread[k] = std::max(mread[k], m + 1);
+    for (int k = 0; k < p->npair; ++k) {
+      const int K = std::min(mmax, p->pairs[k].nphi / 2 + 1);
+      p->pairs[k].mwrite = std::min(mmax, std::max(K, mread[k]));
+    }
   }
 
   std::vector<double> ct(p->kpad, 0.0);

@@ -45,7 +45,8 @@ struct RingPair {
   int chirp_off;  // Bluestein chirp w_j = exp(-i pi j^2 / nphi), nphi entries
   int bhat_off;   // Bluestein kernel spectrum FFT_M(conj w)/M, M entries
   float weight;   // ring quadrature weight
-  float pad_;
+  int mwrite;     // the analysis writes the spectra for m < mwrite (what the
+                  // forward GEMM reads of this pair; zero from min(mmax, nphi/2+1))
 };
 
 struct ShtPlan {

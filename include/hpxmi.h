@@ -225,10 +225,13 @@ int hpx_ring_irfft(const hpx_sht_plan* plan, const float* ftm, float* f, int64_t
  * hpx_sht_stage_ring_analysis followed by hpx_sht_stage_legendre_analysis;
  * hpx_sht_inverse is hpx_sht_stage_legendre_synthesis followed by
  * hpx_sht_stage_ring_synthesis.  A ring pair's spectrum stops at
- * K = min(mmax, nphi / 2 + 1): the ring analysis writes zeros for m >= K,
- * which the Legendre analysis relies on (it skips the ring pairs whose
- * spectrum stops below m), and the Legendre synthesis may leave entries
- * with m >= K unwritten, which the ring synthesis never reads.  No
+ * K = min(mmax, nphi / 2 + 1).  The Legendre analysis skips, per m, the
+ * leading ring pairs whose spectra stop below m, and reads the others
+ * (zeros included) only as far as their spectra are written: the ring
+ * analysis writes each pair for every m that the Legendre analysis reads
+ * (zeros from K on) and leaves the rest unwritten.  The Legendre synthesis
+ * may leave entries with m >= K unwritten, and the ring synthesis never
+ * reads them.  No
  * counterpart in the reference (its stages are the cuhpx_fft calls and the
  * einsums inside one autograd Function); they let callers time, overlap or
  * pipeline the stages. */

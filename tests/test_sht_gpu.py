@@ -9,6 +9,7 @@ import pytest
 import torch
 
 import cuhpx_amd
+from cuhpx_amd import _lib
 from cuhpx_amd.hpx_sht import ShtPlan
 from oracle import sht as S
 from oracle.cpu_path import CPUTorchSHT
@@ -297,3 +298,45 @@ def test_empty_batches_pass_through():
         assert cuhpx_amd.nest2ring(e, nside).shape == (0, npix)
         assert cuhpx_amd.flat2flat(e, "S", False, "N", True, nside).shape == (0, npix)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("nside,lmax,mmax", [(8, 17, 17), (32, 65, 65), (64, 129, 129), (16, 40, 20)])
+def test_no_stage_reads_unwritten_workspace(nside, lmax, mmax):
+    """Each stage reads only workspace entries that an earlier stage of the same
+    call wrote (the ring analysis stops at the last order the Legendre analysis
+    reads of each ring pair, the Legendre synthesis skips ring pairs no ring
+    synthesis reads): with the workspace filled with NaN first, every entry
+    point returns exactly what it returns on a zeroed workspace."""
+    lib = _lib.lib()
+    n = 3
+    plan = ShtPlan.healpix(nside, lmax, mmax)
+    vplan = ShtPlan.healpix_vector(nside, lmax, mmax)
+    s = _lib.stream_ptr(torch.device("cuda"))
+    g = torch.Generator(device="cuda").manual_seed(7)
+    f = torch.randn(n, plan.npix, device="cuda", generator=g)
+    a = torch.randn(n, lmax, mmax, 2, device="cuda", generator=g)
+    fv = torch.randn(n, 2, plan.npix, device="cuda", generator=g)
+    av = torch.randn(n, 2, lmax, mmax, 2, device="cuda", generator=g)
+
+    def run(pl, fn, extra, inp, out_shape, fill):
+        ws, nbytes = pl.workspace(n)
+        ws.view(torch.float32).fill_(fill)  # the workspace is whole floats
+        out = torch.zeros(out_shape, device="cuda")
+        rc = fn(pl.handle, *extra, _lib.ptr(inp), _lib.ptr(out), n, _lib.ptr(ws), nbytes, s)
+        assert rc == 0, lib.hpx_last_error()
+        return out
+
+    cases = [
+        (plan, lib.hpx_sht_forward, (1,), f, (n, lmax, mmax, 2)),
+        (plan, lib.hpx_sht_forward, (0,), f, (n, lmax, mmax, 2)),
+        (plan, lib.hpx_sht_inverse, (1,), a, (n, plan.npix)),
+        (plan, lib.hpx_sht_forward_adjoint, (1, 0), a, (n, plan.npix)),
+        (plan, lib.hpx_sht_inverse_adjoint, (1, 0), f, (n, lmax, mmax, 2)),
+        (vplan, lib.hpx_vsht_forward, (1,), fv, (n, 2, lmax, mmax, 2)),
+        (vplan, lib.hpx_vsht_inverse, (1,), av, (n, 2, plan.npix)),
+    ]
+    for pl, fn, extra, inp, shape in cases:
+        clean = run(pl, fn, extra, inp, shape, 0.0)
+        poisoned = run(pl, fn, extra, inp, shape, float("nan"))
+        assert torch.isfinite(clean).all()
+        assert torch.equal(clean, poisoned), (fn, extra)
