@@ -745,12 +745,15 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
     }
   }
 
-  // rows k0..k0+kRows-1 of X[m][p], this work-group's columns
+  // rows k0..k0+kRows-1 of X[m][p], this work-group's columns; none below
+  // kmin[m], which no synthesis reads at this m
   const int64_t pstride = (int64_t)kpad * ncol;
   float* Xb = X + (int64_t)m * 2 * pstride + (int64_t)k0 * ncol + cb * (64 * CT);
+  const int kfirst = kmin[m];
   store_tiles_lds<CT, RH, true>(acc, reinterpret_cast<float*>(smem), wave, lane, ncol - cb * (64 * CT),
                       [&](int p, int r) -> float* {
-                        return k0 + r < kpad ? Xb + p * pstride + (int64_t)r * ncol : nullptr;
+                        const int k = k0 + r;
+                        return k < kpad && k >= kfirst ? Xb + p * pstride + (int64_t)r * ncol : nullptr;
                       });
 }
 
