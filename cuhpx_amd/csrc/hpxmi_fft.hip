@@ -403,8 +403,9 @@ __device__ __forceinline__ void unpack_pair(float2 zm, float2 zc, float2 ph, flo
 
 // Synthesis input at m for the two rings' spectra Gn, Gs: the DFT below is
 // applied to conj(Z) with Z = Xn + i Xs Hermitian-extended; writes positions
-// m and (unless m is DC or Nyquist) N - m of the sequence (chirp-multiplied
-// for Bluestein).
+// m and (unless m is DC or Nyquist) N - m of the sequence, chirp-multiplied
+// for Bluestein when chirp is given (chirp == nullptr: the first Stockham
+// pass applies it instead).
 template <bool PADDED>
 __device__ __forceinline__ void pack_pair(float2* seq, const RingPair& pr, int m, float2 Gn,
                                           float2 Gs, float2 ph_conj, float c, int cmode,
@@ -418,7 +419,7 @@ __device__ __forceinline__ void pack_pair(float2* seq, const RingPair& pr, int m
     xn.y = 0.f;
     xs.y = 0.f;
   }
-  const bool blue = pr.M != N;
+  const bool blue = pr.M != N && chirp != nullptr;
   float2 zp = cconj(make_float2(xn.x - xs.y, xn.y + xs.x));
   if (blue) zp = cmul(zp, chirp[m]);
   seq[PADDED ? pad(m) : m] = zp;
