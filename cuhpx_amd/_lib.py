@@ -56,6 +56,7 @@ SIGNATURES = {
     "hpx_sht_plan_npix": (_c_i64, [_vp]),
     "hpx_sht_plan_table_bytes": (_c_i64, [_vp]),
     "hpx_sht_workspace_bytes": (_c_i64, [_vp, _c_i64]),
+    "hpx_sht_max_batch": (_c_i64, [_vp]),
     "hpx_sht_forward": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_inverse": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_forward_adjoint": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),

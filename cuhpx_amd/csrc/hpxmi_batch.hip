@@ -54,6 +54,9 @@ int validate(const ShtPlan* p, int precision, int64_t nbatch, int64_t ws_bytes) 
   if (p->lmax < 1) HPX_FAIL(HPX_ERR_ARG, "plan was built with lmax = 0 (ring FFTs only)");
   if (precision < 0 || precision > 2) HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
   if (nbatch < 0) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 0");
+  if (nbatch > max_maps(*p))
+    HPX_FAIL(HPX_ERR_UNSUPPORTED, "%lld maps in one call exceed this plan's limit of %lld; split the batch",
+             (long long)nbatch, (long long)max_maps(*p));
   const int64_t need = workspace_floats(*p, nbatch) * (int64_t)sizeof(float);
   if (ws_bytes < need)
     HPX_FAIL(HPX_ERR_ARG, "workspace of %lld bytes is smaller than the %lld bytes needed",

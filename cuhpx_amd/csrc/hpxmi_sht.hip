@@ -320,6 +320,9 @@ int check_call(const ShtPlan* p, int precision, int64_t nbatch, int64_t ws_bytes
   if (!p->vector && vector_call) HPX_FAIL(HPX_ERR_ARG, "plan holds the scalar table (use hpx_vsht_plan_create_healpix)");
   if (precision < 0 || precision > 2) HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
   if (nbatch < 0) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 0");
+  if (nbatch > max_maps(*p))
+    HPX_FAIL(HPX_ERR_UNSUPPORTED, "%lld maps in one call exceed this plan's limit of %lld; split the batch",
+             (long long)nbatch, (long long)max_maps(*p));
   const int64_t need = workspace_floats(*p, nbatch) * (int64_t)sizeof(float);
   if (ws_bytes < need)
     HPX_FAIL(HPX_ERR_ARG, "workspace of %lld bytes is smaller than the %lld bytes needed",
@@ -446,6 +449,12 @@ extern "C" int64_t hpx_sht_plan_table_bytes(const hpx_sht_plan* plan) {
 extern "C" int64_t hpx_sht_workspace_bytes(const hpx_sht_plan* plan, int64_t nbatch) {
   if (plan == nullptr || nbatch < 0) return -1;
   return workspace_floats(*P(plan), nbatch) * (int64_t)sizeof(float);
+}
+
+extern "C" int64_t hpx_sht_max_batch(const hpx_sht_plan* plan) {
+  if (plan == nullptr) return -1;
+  const ShtPlan* p = P(plan);
+  return p->vector ? max_maps(*p) / 2 : max_maps(*p);
 }
 
 extern "C" int hpx_sht_forward(const hpx_sht_plan* plan, int precision, const float* f, float* alm,

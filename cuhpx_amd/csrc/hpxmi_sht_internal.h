@@ -3,6 +3,7 @@
 
 #include "hpxmi_common.h"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -129,6 +130,14 @@ inline int64_t x_floats(const ShtPlan& p, int64_t nbatch) {
 constexpr int kAtSlackRows = 64;
 inline int64_t workspace_floats(const ShtPlan& p, int64_t nbatch) {
   return x_floats(p, nbatch) + (p.at_rows + kAtSlackRows) * col_pad(nbatch);
+}
+// Most maps one call may carry: the Legendre GEMMs address one order's
+// spectra (2 kpad rows) and coefficient rows (at most lmax + 2 kRowPad) of
+// col_pad(nbatch) floats through 32-bit buffer offsets.
+inline int64_t max_maps(const ShtPlan& p) {
+  const int64_t rows = std::max<int64_t>(2 * (int64_t)p.kpad, (int64_t)p.lmax + 2 * kRowPad);
+  const int64_t max_ncol = 0x7fffffff / (rows * (int64_t)sizeof(float)) / 32 * 32;
+  return max_ncol / 2;
 }
 
 // Launchers (return HPX_OK or an error code after setting the error text).

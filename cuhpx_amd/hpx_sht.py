@@ -51,6 +51,7 @@ class ShtPlan:
         self.lmax = lmax
         self.mmax = mmax
         self.vector = vector
+        self.max_batch = int(_lib.lib().hpx_sht_max_batch(handle))
 
     @property
     def handle(self):
@@ -165,56 +166,60 @@ class ShtPlan:
             rc = fn(self._handle, *args, nbatch, _lib.ptr(ws), nbytes, _lib.stream_ptr(self.device))
         return rc
 
+    def _call(self, fn, head, inp, out, what):
+        """fn(handle, *head, in, out, nbatch, workspace, bytes, stream) over the
+        rows of the contiguous inp and out, at most max_batch rows per call
+        (one call, with nbatch 0, for an empty batch)."""
+        n = inp.shape[0]
+        step = max(1, self.max_batch)
+        ib = inp.stride(0) * inp.element_size()
+        ob = out.stride(0) * out.element_size()
+        for s in range(0, max(n, 1), step):
+            c = min(step, n - s)
+            rc = self._run(fn, *head, ctypes.c_void_p(inp.data_ptr() + s * ib),
+                           ctypes.c_void_p(out.data_ptr() + s * ob), nbatch=c)
+            _lib.check(rc, what)
+
     def forward(self, f, precision):
         """f [n, npix] float32 contiguous -> alm [n, lmax, mmax] complex64."""
         n = f.shape[0]
         out = torch.empty((n, self.lmax, self.mmax), dtype=torch.complex64, device=f.device)
-        rc = self._run(_lib.lib().hpx_sht_forward, _lib.PRECISIONS[precision], _lib.ptr(f),
-                       _lib.ptr(out), nbatch=n)
-        _lib.check(rc, "SHT forward")
+        self._call(_lib.lib().hpx_sht_forward, (_lib.PRECISIONS[precision],), f, out, "SHT forward")
         return out
 
     def inverse(self, alm, precision):
         """alm [n, lmax, mmax] complex64 contiguous -> f [n, npix] float32."""
         n = alm.shape[0]
         out = torch.empty((n, self.npix), dtype=torch.float32, device=alm.device)
-        rc = self._run(_lib.lib().hpx_sht_inverse, _lib.PRECISIONS[precision], _lib.ptr(alm),
-                       _lib.ptr(out), nbatch=n)
-        _lib.check(rc, "SHT inverse")
+        self._call(_lib.lib().hpx_sht_inverse, (_lib.PRECISIONS[precision],), alm, out, "SHT inverse")
         return out
 
     def vector_forward(self, f, precision):
         """f [n, 2, npix] float32 contiguous -> alm [n, 2, lmax, mmax] complex64."""
         n = f.shape[0]
         out = torch.empty((n, 2, self.lmax, self.mmax), dtype=torch.complex64, device=f.device)
-        rc = self._run(_lib.lib().hpx_vsht_forward, _lib.PRECISIONS[precision], _lib.ptr(f),
-                       _lib.ptr(out), nbatch=n)
-        _lib.check(rc, "vector SHT forward")
+        self._call(_lib.lib().hpx_vsht_forward, (_lib.PRECISIONS[precision],), f, out, "vector SHT forward")
         return out
 
     def vector_inverse(self, alm, precision):
         """alm [n, 2, lmax, mmax] complex64 contiguous -> f [n, 2, npix] float32."""
         n = alm.shape[0]
         out = torch.empty((n, 2, self.npix), dtype=torch.float32, device=alm.device)
-        rc = self._run(_lib.lib().hpx_vsht_inverse, _lib.PRECISIONS[precision], _lib.ptr(alm),
-                       _lib.ptr(out), nbatch=n)
-        _lib.check(rc, "vector SHT inverse")
+        self._call(_lib.lib().hpx_vsht_inverse, (_lib.PRECISIONS[precision],), alm, out, "vector SHT inverse")
         return out
 
     def forward_adjoint(self, galm, precision, mode):
         n = galm.shape[0]
         out = torch.empty((n, self.npix), dtype=torch.float32, device=galm.device)
-        rc = self._run(_lib.lib().hpx_sht_forward_adjoint, _lib.PRECISIONS[precision],
-                       _lib.ADJOINT_MODES[mode], _lib.ptr(galm), _lib.ptr(out), nbatch=n)
-        _lib.check(rc, "SHT forward adjoint")
+        self._call(_lib.lib().hpx_sht_forward_adjoint, (_lib.PRECISIONS[precision], _lib.ADJOINT_MODES[mode]),
+                   galm, out, "SHT forward adjoint")
         return out
 
     def inverse_adjoint(self, gf, precision, mode):
         n = gf.shape[0]
         out = torch.empty((n, self.lmax, self.mmax), dtype=torch.complex64, device=gf.device)
-        rc = self._run(_lib.lib().hpx_sht_inverse_adjoint, _lib.PRECISIONS[precision],
-                       _lib.ADJOINT_MODES[mode], _lib.ptr(gf), _lib.ptr(out), nbatch=n)
-        _lib.check(rc, "SHT inverse adjoint")
+        self._call(_lib.lib().hpx_sht_inverse_adjoint, (_lib.PRECISIONS[precision], _lib.ADJOINT_MODES[mode]),
+                   gf, out, "SHT inverse adjoint")
         return out
 
 

@@ -133,6 +133,13 @@ int64_t hpx_sht_plan_table_bytes(const hpx_sht_plan* plan);
  * maps need; the caller allocates it (256-byte aligned). */
 int64_t hpx_sht_workspace_bytes(const hpx_sht_plan* plan, int64_t nbatch);
 
+/* Largest nbatch that one call on this plan accepts (maps for a scalar plan,
+ * fields for a vector plan): the Legendre GEMMs address one order's data
+ * through 32-bit offsets, max(2 * ring pairs, lmax + 32) rows of 2 * nbatch
+ * floats.  Larger calls return HPX_ERR_UNSUPPORTED; split the batch.  No
+ * counterpart in the reference. */
+int64_t hpx_sht_max_batch(const hpx_sht_plan* plan);
+
 /* Forward SHT, f[nbatch, npix] -> alm[nbatch, lmax, mmax] (complex64).
  * Replaces SHTFunction.forward (hpx_sht.py:517-562): cuhpx_fft
  * healpix_rfft_batch / healpix_rfft_class (hpx_fft.cpp:21-84, 152-193)

@@ -70,6 +70,7 @@ def test_plan_rejects_bad_arguments():
     # compute calls validate the plan and precision before any launch
     assert lib.hpx_sht_forward(None, 0, None, None, 1, None, 0, None) == _lib.HPX_ERR_ARG
     assert lib.hpx_sht_workspace_bytes(None, 1) == -1
+    assert lib.hpx_sht_max_batch(None) == -1
 
 
 def test_check_maps_codes_to_exceptions():

@@ -340,3 +340,28 @@ def test_no_stage_reads_unwritten_workspace(nside, lmax, mmax):
         poisoned = run(pl, fn, extra, inp, shape, float("nan"))
         assert torch.isfinite(clean).all()
         assert torch.equal(clean, poisoned), (fn, extra)
+
+
+def test_batches_above_the_per_call_limit_are_split():
+    """The library takes at most hpx_sht_max_batch maps per call (32-bit
+    offsets in the GEMMs) and refuses more before touching memory; the
+    modules split larger batches, with results identical to one call."""
+    nside, L = 8, 17
+    plan = ShtPlan.healpix(nside, L, L)
+    lib = _lib.lib()
+    assert plan.max_batch == lib.hpx_sht_max_batch(plan.handle)
+    # kpad = 32 ring pairs, lmax + 32 = 49 rows: 2^31 / (64 rows * 4 bytes), rounded, halved
+    assert plan.max_batch == (0x7FFFFFFF // (64 * 4)) // 32 * 32 // 2
+    s = _lib.stream_ptr(torch.device("cuda"))
+    rc = lib.hpx_sht_forward(plan.handle, 1, None, None, plan.max_batch + 1, None, 0, s)
+    assert rc == _lib.HPX_ERR_UNSUPPORTED and b"split" in lib.hpx_last_error()
+    f = torch.randn(40, plan.npix, device="cuda")
+    a = plan.forward(f, "bf16x3")
+    back = plan.inverse(a, "bf16x3")
+    saved = plan.max_batch
+    try:
+        plan.max_batch = 16
+        assert torch.equal(plan.forward(f, "bf16x3"), a)
+        assert torch.equal(plan.inverse(a, "bf16x3"), back)
+    finally:
+        plan.max_batch = saved
