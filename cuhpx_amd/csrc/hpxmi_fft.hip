@@ -566,10 +566,12 @@ __global__ __launch_bounds__(WAVES * TPF) void ring_synthesis_wave_kernel(const 
   const int K = min(a.mmax, N / 2 + 1);
   // folded spectra (S, D) of element e = (m = e / WAVES, field g = e % WAVES),
   // kBatch elements per thread per batch; the first batch is requested
-  // before the LDS set-up and each next batch while the current one is packed
+  // before the LDS set-up and each next batch while the current one is
+  // packed.  One element per batch measured fastest (ring synthesis at nside
+  // 256, 64 fields: 190 us with 4, 185 with 2, 177 with 1; 219 with 8).
   const int64_t pstride = (int64_t)a.kpad * a.ncol;
   const int total = K * WAVES;
-  constexpr int kBatch = 4;
+  constexpr int kBatch = 1;
   float2 S[kBatch], D[kBatch];
   auto load_batch = [&](int e0) {
 #pragma unroll
