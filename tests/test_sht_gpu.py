@@ -300,23 +300,11 @@ def test_empty_batches_pass_through():
     torch.cuda.synchronize()
 
 
-@pytest.mark.parametrize("nside,lmax,mmax", [(8, 17, 17), (32, 65, 65), (64, 129, 129), (16, 40, 20)])
-def test_no_stage_reads_unwritten_workspace(nside, lmax, mmax):
-    """Each stage reads only workspace entries that an earlier stage of the same
-    call wrote (the ring analysis stops at the last order the Legendre analysis
-    reads of each ring pair, the Legendre synthesis skips ring pairs no ring
-    synthesis reads): with the workspace filled with NaN first, every entry
-    point returns exactly what it returns on a zeroed workspace."""
+def _poisoned_equals_clean(cases, n):
+    """Each (plan, entry point, leading args, input, output shape) gives the same
+    result on a NaN-filled workspace as on a zeroed one."""
     lib = _lib.lib()
-    n = 3
-    plan = ShtPlan.healpix(nside, lmax, mmax)
-    vplan = ShtPlan.healpix_vector(nside, lmax, mmax)
     s = _lib.stream_ptr(torch.device("cuda"))
-    g = torch.Generator(device="cuda").manual_seed(7)
-    f = torch.randn(n, plan.npix, device="cuda", generator=g)
-    a = torch.randn(n, lmax, mmax, 2, device="cuda", generator=g)
-    fv = torch.randn(n, 2, plan.npix, device="cuda", generator=g)
-    av = torch.randn(n, 2, lmax, mmax, 2, device="cuda", generator=g)
 
     def run(pl, fn, extra, inp, out_shape, fill):
         ws, nbytes = pl.workspace(n)
@@ -326,20 +314,56 @@ def test_no_stage_reads_unwritten_workspace(nside, lmax, mmax):
         assert rc == 0, lib.hpx_last_error()
         return out
 
-    cases = [
-        (plan, lib.hpx_sht_forward, (1,), f, (n, lmax, mmax, 2)),
-        (plan, lib.hpx_sht_forward, (0,), f, (n, lmax, mmax, 2)),
-        (plan, lib.hpx_sht_inverse, (1,), a, (n, plan.npix)),
-        (plan, lib.hpx_sht_forward_adjoint, (1, 0), a, (n, plan.npix)),
-        (plan, lib.hpx_sht_inverse_adjoint, (1, 0), f, (n, lmax, mmax, 2)),
-        (vplan, lib.hpx_vsht_forward, (1,), fv, (n, 2, lmax, mmax, 2)),
-        (vplan, lib.hpx_vsht_inverse, (1,), av, (n, 2, plan.npix)),
-    ]
     for pl, fn, extra, inp, shape in cases:
         clean = run(pl, fn, extra, inp, shape, 0.0)
         poisoned = run(pl, fn, extra, inp, shape, float("nan"))
         assert torch.isfinite(clean).all()
         assert torch.equal(clean, poisoned), (fn, extra)
+
+
+def _scalar_cases(plan, n, g):
+    lib = _lib.lib()
+    L, M = plan.lmax, plan.mmax
+    f = torch.randn(n, plan.npix, device="cuda", generator=g)
+    a = torch.randn(n, L, M, 2, device="cuda", generator=g)
+    return [
+        (plan, lib.hpx_sht_forward, (1,), f, (n, L, M, 2)),
+        (plan, lib.hpx_sht_forward, (0,), f, (n, L, M, 2)),
+        (plan, lib.hpx_sht_inverse, (1,), a, (n, plan.npix)),
+        (plan, lib.hpx_sht_forward_adjoint, (1, 0), a, (n, plan.npix)),
+        (plan, lib.hpx_sht_inverse_adjoint, (1, 0), f, (n, L, M, 2)),
+    ]
+
+
+@pytest.mark.parametrize("nside,lmax,mmax", [(8, 17, 17), (32, 65, 65), (64, 129, 129), (16, 40, 20)])
+def test_no_stage_reads_unwritten_workspace(nside, lmax, mmax):
+    """Each stage reads only workspace entries that an earlier stage of the same
+    call wrote (the ring analysis stops at the last order the Legendre analysis
+    reads of each ring pair, the Legendre synthesis skips ring pairs no ring
+    synthesis reads): with the workspace filled with NaN first, every entry
+    point returns exactly what it returns on a zeroed workspace."""
+    lib = _lib.lib()
+    n = 3
+    g = torch.Generator(device="cuda").manual_seed(7)
+    plan = ShtPlan.healpix(nside, lmax, mmax)
+    vplan = ShtPlan.healpix_vector(nside, lmax, mmax)
+    fv = torch.randn(n, 2, plan.npix, device="cuda", generator=g)
+    av = torch.randn(n, 2, lmax, mmax, 2, device="cuda", generator=g)
+    _poisoned_equals_clean(_scalar_cases(plan, n, g) + [
+        (vplan, lib.hpx_vsht_forward, (1,), fv, (n, 2, lmax, mmax, 2)),
+        (vplan, lib.hpx_vsht_inverse, (1,), av, (n, 2, plan.npix)),
+    ], n)
+
+
+def test_no_stage_reads_unwritten_workspace_latlon():
+    """The same on a lat/lon plan whose spectra stop below mmax on every ring
+    (nlon 16: K = 9 < mmax = 12), with an odd ring count (a self-mirrored
+    equator) and 11 ring pairs (not a multiple of the 32-pair chunk)."""
+    from cuhpx_amd.hpx_regrid import latlon_plan
+    n = 3
+    plan = latlon_plan("legendre-gauss", 21, 16, 12, 12, csphase=True)
+    g = torch.Generator(device="cuda").manual_seed(8)
+    _poisoned_equals_clean(_scalar_cases(plan, n, g), n)
 
 
 def test_batches_above_the_per_call_limit_are_split():
