@@ -376,6 +376,9 @@ def test_batches_above_the_per_call_limit_are_split():
     assert plan.max_batch == lib.hpx_sht_max_batch(plan.handle)
     # kpad = 32 ring pairs, lmax + 32 = 49 rows: 2^31 / (64 rows * 4 bytes), rounded, halved
     assert plan.max_batch == (0x7FFFFFFF // (64 * 4)) // 32 * 32 // 2
+    # a vector plan counts fields, two maps each
+    vplan = ShtPlan.healpix_vector(nside, L, L)
+    assert vplan.max_batch == lib.hpx_sht_max_batch(vplan.handle) == plan.max_batch // 2
     s = _lib.stream_ptr(torch.device("cuda"))
     rc = lib.hpx_sht_forward(plan.handle, 1, None, None, plan.max_batch + 1, None, 0, s)
     assert rc == _lib.HPX_ERR_UNSUPPORTED and b"split" in lib.hpx_last_error()
