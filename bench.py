@@ -181,23 +181,28 @@ class Runner:
         return (t1 - t0) * 1e3 / steps, stage_ms
 
 
-def cpu_baseline(nmaps=3):
+def cpu_baseline(min_seconds=10.0, min_maps=3, max_maps=64):
     """The reference's CPU torch SHT path (restated in oracle/cpu_path.py) on
-    nmaps maps of the same workload, after one warm-up map."""
+    the same workload, one map at a time after one warm-up map, until at least
+    min_seconds of timed work (and min_maps maps, at most max_maps) are done;
+    the per-map time is the median."""
     from oracle.cpu_path import CPUTorchSHT
     threads = os.cpu_count() or 1
     torch.set_num_threads(threads)
     t0 = time.perf_counter()
     cpu = CPUTorchSHT(NSIDE, LMAX, MMAX)
     build_s = time.perf_counter() - t0
-    alm = band_limited_alm(LMAX, MMAX, nmaps + 1, 1).to(torch.complex64)
+    alm = band_limited_alm(LMAX, MMAX, 4, 1).to(torch.complex64)
     times = []
-    for i in range(nmaps + 1):
+    for i in range(max_maps + 1):
         t0 = time.perf_counter()
-        f = cpu.inverse(alm[i].clone())
+        f = cpu.inverse(alm[i % 4].clone())
         cpu.forward(f)
         if i > 0:
             times.append(time.perf_counter() - t0)
+            if len(times) >= min_maps and sum(times) >= min_seconds:
+                break
+    nmaps = len(times)
     per_map = float(np.median(times))
     cpu_model = platform.processor() or ""
     try:
@@ -213,7 +218,8 @@ def cpu_baseline(nmaps=3):
         "unit": "maps/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"{nmaps} maps (after 1 warm-up), forward+inverse each, median; reference CPU torch "
+        "sample": (f"{nmaps} maps ({sum(times):.1f} s timed, after 1 warm-up), forward+inverse each, median; "
+                   f"reference CPU torch "
                    f"path (per-ring torch.fft + einsum, float32) at nside={NSIDE}, lmax=mmax={LMAX}; "
                    f"P-table build {build_s:.1f} s not timed; host '{cpu_model}'"),
     }, cpu
