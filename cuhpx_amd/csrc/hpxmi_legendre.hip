@@ -689,10 +689,12 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
       for (int q = 0; q < 16; ++q) L.hi[sp][sr][shs + q] = (__bf16)v[q];
     }
   };
-  // B fragments: rows i of aT (this m's rows of parity p).  Rows beyond the
-  // parity's count are zero padding, the other parity's or the next m's
-  // rows, or the zero slack after aT: finite values that meet zero table
-  // entries (PT is zero for i >= the padded count), so no bound check.
+  // B fragments: rows i of aT (this m's rows of parity p).  Rows at or
+  // beyond the parity's count (its padding, the other parity's or the next
+  // m's rows, or the slack after aT) meet zero table entries; they are read
+  // in bounds of the workspace and set to zero in the parity's last chunk,
+  // so whatever they hold never reaches the products.
+  const int kl = frag_k_lane<PREC>(lane);
   auto load_b = [&](int ic, float (&bv)[2][CT][8]) {
     const bool half1 = VEC && ic >= kp;
     const int ii = half1 ? ic - kp : ic;
@@ -706,6 +708,13 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
         for (int q = 0; q < 8; ++q)
           bv[p][ct][q] = buffer_load(
               ar, voff, (int)(((int64_t)((pb ? padp[0] : 0) + ii + frag_k_uniform<PREC>(q)) * nc + ct * 16) * 4));
+      if (ii + kKTile > nlp[pb]) {
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+          for (int q = 0; q < 8; ++q)
+            if (ii + kl + frag_k_uniform<PREC>(q) >= nlp[pb]) bv[p][ct][q] = 0.f;
+      }
     }
     if constexpr (VEC) {
       const float sg = half1 ? sgn1 : sgn0;
@@ -848,33 +857,6 @@ __global__ __launch_bounds__(kThreads) void coeff_to_user_kernel(
     const float2 v = zero ? make_float2(0.f, 0.f) : make_float2(sc * t[mm][2 * bs], sc * t[mm][2 * bs + 1]);
     *reinterpret_cast<float2*>(alm + 2 * ((b * lmax + l) * (int64_t)mmax + m)) = v;
   }
-}
-
-// Zero the padding rows of aT (rows [nl, padded count) of each (m, parity))
-// and the slack rows after the last one; block = (m, parity), the last
-// block also clears the slack.
-__global__ __launch_bounds__(kThreads) void zero_coeff_pad_kernel(float* __restrict__ aT,
-                                                                  const int64_t* __restrict__ p_off,
-                                                                  const int* __restrict__ nl, int kpad,
-                                                                  int mmax, int64_t at_rows, int64_t ncol) {
-  const int m = blockIdx.x >> 1;
-  const int par = blockIdx.x & 1;
-  int64_t r0, r1;
-  if (m < mmax) {
-    const int n0 = nl[2 * m];
-    const int pad0 = (n0 + kRowPad - 1) / kRowPad * kRowPad;
-    const int n = nl[2 * m + par];
-    const int pad = (n + kRowPad - 1) / kRowPad * kRowPad;
-    const int64_t base = p_off[m] / kpad + (par ? pad0 : 0);
-    r0 = base + n;
-    r1 = base + pad;
-  } else {
-    r0 = at_rows;
-    r1 = at_rows + kAtSlackRows;
-  }
-  const int64_t n = (r1 - r0) * ncol;
-  float* dst = aT + r0 * ncol;
-  for (int64_t i = threadIdx.x; i < n; i += kThreads) dst[i] = 0.f;
 }
 
 // aT from a[b][l][m] (rows m <= l only); columns of fields >= nbatch are zero.
@@ -1062,9 +1044,6 @@ int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float
   if (precision < 0 || precision > 2) HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
   float* aT = ws + x_floats(p, nbatch);
   const GemmTune t = p.gemm_tune(precision, col_pad(nbatch));
-  zero_coeff_pad_kernel<<<(unsigned)(2 * p.mmax + 1), kThreads, 0, s>>>(aT, p.d_p_off, p.d_nl, p.kpad, p.mmax,
-                                                                        p.at_rows, col_pad(nbatch));
-  HPX_LAUNCH_CHECK();
   int rc = launch_user_to_coeff(p, alm, aT, nbatch, s);
   if (rc != HPX_OK) return rc;
   return launch_gemm_inv(p, precision, t.inv_ct, t.inv_rh, aT, ws, nbatch, s);
