@@ -43,7 +43,11 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 constexpr int kThreads = 256;
 constexpr int kRowsPerParity = 64;  // forward: rows i per parity per work-group
 constexpr int kKBlock = 64;         // inverse: ring pairs per work-group
-constexpr int kLdsPadF = 1;         // FP32 LDS row padding (floats)
+// FP32 tile rows of 36 floats: row i starts at bank 36 i mod 64, sixteen
+// distinct multiples of 4, so the fragment reads (row lane & 15, column
+// 4 q + lane >> 4) hit all 64 banks once, and rows stay 16-byte aligned for
+// the staging stores (33-float rows: 2-way conflicts, 4-byte stores).
+constexpr int kLdsPadF = 4;
 
 __device__ __forceinline__ void split(float x, __bf16& hi, __bf16& lo) {
   hi = (__bf16)x;
