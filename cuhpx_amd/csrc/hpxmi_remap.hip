@@ -16,7 +16,9 @@
 // source and destination index of its tile's pixels once, in registers, and
 // then streams the batch rows through LDS: the source side is read in its
 // own natural order (coalesced), the destination side is written in its own
-// natural order (coalesced).
+// natural order (coalesced).  NEST -> RING at nside >= 512 uses tiles that
+// follow the rings instead (nest2ring_bands), so that its RING writes come
+// in runs of 1-2 KB.
 
 #include "hpxmi_common.h"
 #include "../../include/hpxmi.h"
@@ -255,6 +257,139 @@ __global__ __launch_bounds__(TBMAX * TBMAX / kPerThread) void remap_tiles(
   }
 }
 
+// NEST -> RING through tiles that follow the rings.  A square tile's ring
+// segments average TB/2 pixels, and a RING destination written in such short
+// runs is slow (nest2ring 3.3 TB/s at nside 1024, against 5.2 TB/s for the
+// kinds whose destination is NEST or flat).  Here a tile is a parallelogram
+// of one face: the BS rings (anti-diagonals s = ix + iy) in [s0, s0 + BS),
+// each cut to the BW pixels with ix in [x0, x0 + BW).  Each of its rings is
+// one contiguous RING run of up to BW pixels (two where the ring wraps round
+// the pixel-index origin of face 4), written with non-temporal stores: with
+// runs of 1-2 KB few lines are partial, and the stores then no longer need
+// the L2 to merge them (plain stores: 3.4 TB/s).  The NEST side is read as
+// whole 8 x 8 Morton blocks (256 contiguous bytes at 4-byte pixels, one
+// wavefront each): for every column of blocks the band meets BS / 8 + 1
+// blocks, of which the first and last lie partly in the neighbouring bands
+// and are masked.  Along ix a face holds nside / BW tiles; along s, the band
+// of an x-range covers s in [x0, x0 + BW + nside - 2], i.e. (BW + nside) / BS
+// tiles, the ones at either end only partly filled.  LDS holds the tile as
+// BS rows of BW pixels with a pitch of BW + 8: an 8 x 8 block's 64 pixels
+// then fall in 64 distinct banks.  Measured at [64, npix] float32:
+// 512 x 32 tiles 1.47 ms at nside 1024 (256 x 64: 1.56 ms; 1024 x 16 spills
+// registers: 2.3 ms); at nside 512, where 512 x 32 tiles are half empty,
+// 256 x 64 tiles take 0.40 ms against 0.46 ms.
+constexpr int kBandThreads = 1024;
+
+__device__ __forceinline__ uint32_t band_slot(const uint32_t* q, int i) {
+  return (q[i / 2] >> (16 * (i & 1))) & 0xffff;
+}
+
+template <typename T, int BW, int BS>
+__global__ __launch_bounds__(kBandThreads) void nest2ring_bands(
+    const T* __restrict__ in, T* __restrict__ out, int64_t nside, int xblocks, int sblocks,
+    int64_t nrows, int rows_per_chunk) {
+  constexpr int kPitch = BW + 8;
+  constexpr int kBlocksPerCol = BS / 8 + 1;
+  constexpr int kRingPer = BS * BW / kBandThreads;                   // RING pixels per thread
+  constexpr int kNestPer = (BW / 8) * kBlocksPerCol * 64 / kBandThreads;  // NEST slots per thread
+  static_assert(kRingPer % 2 == 0 && kNestPer % 2 == 0 && BS * kPitch < 0xffff, "band shape");
+  __shared__ T lds[2][BS * kPitch];
+  const int n = (int)nside;
+  const int tile = xcd_local_index(blockIdx.x, gridDim.x);
+  const int per_face = xblocks * sblocks;
+  const int face = tile / per_face;
+  const int rem = tile - face * per_face;
+  const int xb = rem / sblocks;
+  const int x0 = xb * BW;
+  const int s0 = x0 + (rem - xb * sblocks) * BS;
+
+  // Source: NEST, whole 8 x 8 Morton blocks.  Block (bx, by) covers
+  // s in [8 (bx + by), 8 (bx + by) + 14]; it meets the band for
+  // bx + by in [s0 / 8 - 1, s0 / 8 + BS / 8 - 1].
+  // LDS slots (< BS * kPitch) are kept two to a register, 0xffff marking a
+  // slot outside the band: 1024 threads leave 128 VGPRs each.
+  uint32_t src_q[kNestPer / 2] = {}, dst_q[kRingPer / 2] = {};
+  int src_i[kNestPer];
+  const int face_base = face * n * n;  // < 2^31 for nside <= 8192
+#pragma unroll
+  for (int i = 0; i < kNestPer; ++i) {
+    const int e = threadIdx.x + i * kBandThreads;
+    const int blk = e >> 6;
+    const int l = e & 63;
+    const int col = blk / kBlocksPerCol;
+    const int bx = (x0 >> 3) + col;
+    const int by = (s0 >> 3) - 1 - bx + (blk - col * kBlocksPerCol);
+    const int ix = 8 * bx + compress_bits(l);
+    const int iy = 8 * by + compress_bits(l >> 1);
+    const int t = ix + iy - s0;
+    const bool ok = by >= 0 && by < (n >> 3) && t >= 0 && t < BS;
+    src_q[i / 2] |= (uint32_t)(ok ? t * kPitch + (ix - x0) : 0xffff) << (16 * (i & 1));
+    src_i[i] = ok ? face_base + (((spread_bits(bx) | (spread_bits(by) << 1)) << 6) | l) : 0;
+  }
+  // Destination: RING, ring t of the band, pixel p along it (increasing ix).
+  int dst_i[kRingPer];
+#pragma unroll
+  for (int i = 0; i < kRingPer; ++i) {
+    const int e = threadIdx.x + i * kBandThreads;
+    const int t = e / BW;
+    const int p = e - t * BW;
+    const int ix = x0 + p;
+    const int iy = s0 + t - ix;
+    const bool ok = iy >= 0 && iy < n;
+    dst_q[i / 2] |= (uint32_t)(ok ? t * kPitch + p : 0xffff) << (16 * (i & 1));
+    dst_i[i] = ok ? (int)xyf2ring(ix, iy, face, nside) : 0;
+  }
+
+  const int64_t npix = 12 * nside * nside;
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+  int64_t r1 = r0 + rows_per_chunk;
+  if (r1 > nrows) r1 = nrows;
+  int buf = 0;
+  for (int64_t r = r0; r < r1; ++r) {
+    const T* src_row = in + r * npix;
+    T* dst_row = out + r * npix;
+    T v[kNestPer];
+#pragma unroll
+    for (int i = 0; i < kNestPer; ++i)
+      if (band_slot(src_q, i) != 0xffff) v[i] = src_row[src_i[i]];
+#pragma unroll
+    for (int i = 0; i < kNestPer; ++i) {
+      const uint32_t q = band_slot(src_q, i);
+      if (q != 0xffff) lds[buf][q] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kRingPer; ++i) {
+      const uint32_t q = band_slot(dst_q, i);
+      if (q != 0xffff) __builtin_nontemporal_store(lds[buf][q], dst_row + dst_i[i]);
+    }
+    buf ^= 1;
+  }
+}
+
+template <typename T, int BW, int BS>
+int launch_bands(const void* in, void* out, int64_t nside, int64_t nrows, hipStream_t stream) {
+  if (nside < BW) HPX_FAIL(HPX_ERR_UNSUPPORTED, "band tiles need nside >= %d", BW);
+  const int xblocks = (int)(nside / BW);
+  const int sblocks = (int)((BW + nside) / BS);
+  const int ntiles = 12 * xblocks * sblocks;
+  // Enough work-groups to fill 256 CUs (one ~132 KB work-group each) about
+  // six times: two row chunks at nside 1024 (one chunk: 1.49 ms, three:
+  // 1.48 ms, two: 1.47 ms).
+  const int64_t want = 1536;
+  int64_t chunks = (want + ntiles - 1) / ntiles;
+  if (chunks > nrows) chunks = nrows;
+  if (chunks < 1) chunks = 1;
+  const int rows_per_chunk = (int)((nrows + chunks - 1) / chunks);
+  chunks = (nrows + rows_per_chunk - 1) / rows_per_chunk;
+  dim3 grid(ntiles, (unsigned)chunks);
+  nest2ring_bands<T, BW, BS><<<grid, kBandThreads, 0, stream>>>(
+      static_cast<const T*>(in), static_cast<T*>(out), nside, xblocks, sblocks, nrows,
+      rows_per_chunk);
+  HPX_LAUNCH_CHECK();
+  return HPX_OK;
+}
+
 __global__ __launch_bounds__(256) void copy16(const uint4* __restrict__ in,
                                               uint4* __restrict__ out, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -310,6 +445,10 @@ int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Si
   const bool ring = src.scheme == kRing || dst.scheme == kRing;
   const bool nest_flat = (src.scheme == kNest && dst.scheme == kFlat) || (src.scheme == kFlat && dst.scheme == kNest);
   if constexpr (sizeof(T) <= 4) {
+    if (src.scheme == kNest && dst.scheme == kRing) {
+      if (nside >= 1024) return launch_bands<T, 512, 32>(in, out, nside, nrows, stream);
+      if (nside == 512) return launch_bands<T, 256, 64>(in, out, nside, nrows, stream);
+    }
     if (ring && nside >= 128) {
       // RING -> NEST / flat: the destination streams whole lines, stored
       // non-temporally (ring2nest 1.33 -> 1.25 ms at nside 1024, 64 rows);

@@ -144,6 +144,21 @@ def test_full_size_nside1024_batch64(kind, args):
     assert torch.equal(of, torch.gather(xf, 1, oi.long()))
 
 
+@pytest.mark.parametrize("nside,rows,dtype", [(512, 3, torch.int32), (512, 2, torch.int16),
+                                              (512, 2, torch.uint8), (1024, 2, torch.int16),
+                                              (1024, 2, torch.uint8), (2048, 2, torch.int32)])
+def test_nest2ring_band_tiles(nside, rows, dtype):
+    """nest2ring at nside >= 512 with pixels of at most 4 bytes walks the map in
+    parallelogram tiles that follow the rings (32 rings x 512 pixels from
+    nside 1024, 64 x 256 at nside 512, partly filled at each face's corners):
+    bit-exact against the oracle for every element size of both tile shapes,
+    and at one nside above the measured size."""
+    print(f"[band tiles] nest2ring nside={nside} rows={rows} {dtype}", flush=True)
+    x = _payload(nside, rows).to(dtype)
+    got = cuhpx_amd.nest2ring(x.cuda(), nside).cpu().numpy()
+    np.testing.assert_array_equal(got, _expected("nest2ring", x.numpy(), nside))
+
+
 def test_full_size_nside1024_all_origins():
     """Origin coverage at the measured size ([64, 12*1024^2] int32 index rows):
     nest2flat for all four origins in both directions against the oracle
