@@ -159,6 +159,27 @@ def test_nest2ring_band_tiles(nside, rows, dtype):
     np.testing.assert_array_equal(got, _expected("nest2ring", x.numpy(), nside))
 
 
+@pytest.mark.parametrize("nside", [512, 1024])
+def test_flat2ring_large_nside_all_origins(nside):
+    """flat2ring (a rotated / mirrored flat source, a RING destination) at the
+    sizes where the RING-kind tiles are 128 pixels wide: bit-exact against the
+    oracle for all four origins in both directions, plus 1- and 2-byte pixels
+    for one orientation."""
+    x = _payload(nside, 2)
+    x32 = x.to(torch.int32)
+    xd = x32.cuda()
+    for o in ORIGINS:
+        for cw in (False, True):
+            print(f"[large nside] flat2ring nside={nside} origin={o} cw={cw}", flush=True)
+            got = _call("flat2ring", xd, nside, o, cw).cpu().numpy()
+            np.testing.assert_array_equal(got, _expected("flat2ring", x32.numpy(), nside, o, cw),
+                                          err_msg=f"flat2ring nside={nside} origin={o} cw={cw}")
+    for dtype in (torch.int16, torch.uint8):
+        xs = x.to(dtype)
+        got = _call("flat2ring", xs.cuda(), nside, "W", True).cpu().numpy()
+        np.testing.assert_array_equal(got, _expected("flat2ring", xs.numpy(), nside, "W", True))
+
+
 def test_full_size_nside1024_all_origins():
     """Origin coverage at the measured size ([64, 12*1024^2] int32 index rows):
     nest2flat for all four origins in both directions against the oracle
