@@ -225,19 +225,31 @@ def cpu_baseline(min_seconds=10.0, min_maps=3, max_maps=64):
     }, cpu
 
 
-def accuracy(cpu, f32_in, alm_in, outputs):
-    """rel-L2 of GPU outputs vs the FP64 oracle on the first fields."""
+ACC_TOL = {"fp32": 1e-5, "bf16x3": 2e-5, "bf16": 1e-2}
+
+
+def accuracy(cpu, f32_in, outputs):
+    """rel-L2 against the FP64 oracle of every field of the timed run: the
+    forward output alm = SHT(f) against the oracle's SHT of the same FP32
+    input, and the inverse output out = iSHT(alm) against the oracle's iSHT
+    of those (FP32) coefficients.  Per precision: the largest and the
+    median per-field error of each direction, and whether the largest stays
+    within the precision's tolerance."""
     from oracle.sht import SHTOracle
     o = SHTOracle(cpu.grid, LMAX, MMAX, pct=cpu.pct.numpy())
     f = f32_in.cpu().numpy().astype(np.float64)
     a_ref = o.forward(f)
-    a = alm_in.cpu().numpy().astype(np.complex128)
-    f_ref = o.inverse(a)
     res = {}
     for name, (alm_out, f_out) in outputs.items():
-        ea = np.linalg.norm(alm_out.cpu().numpy() - a_ref) / np.linalg.norm(a_ref)
-        ef = np.linalg.norm(f_out.cpu().numpy() - f_ref) / np.linalg.norm(f_ref)
-        res[name] = {"forward_rel_l2": float(ea), "inverse_rel_l2": float(ef)}
+        a = alm_out.cpu().numpy()
+        g_ref = o.inverse(a.astype(np.complex128))
+        g = f_out.cpu().numpy()
+        ea = [float(np.linalg.norm(a[i] - a_ref[i]) / np.linalg.norm(a_ref[i])) for i in range(len(a))]
+        ef = [float(np.linalg.norm(g[i] - g_ref[i]) / np.linalg.norm(g_ref[i])) for i in range(len(g))]
+        res[name] = {"fields": len(a), "forward_rel_l2": max(ea), "inverse_rel_l2": max(ef),
+                     "forward_rel_l2_median": float(np.median(ea)),
+                     "inverse_rel_l2_median": float(np.median(ef)),
+                     "within_tolerance": max(ea + ef) <= ACC_TOL[name], "tolerance": ACC_TOL[name]}
     return res
 
 
@@ -569,13 +581,13 @@ def main():
         if cpu is None:
             from oracle.cpu_path import CPUTorchSHT
             cpu = CPUTorchSHT(NSIDE, LMAX, MMAX)
-        nchk = 2
-        outs = {}
-        for prec, (_, _, r) in runs.items():
-            # the last timed step left alm = forward(f) and out = inverse(alm)
-            inv_of_ref = plan.inverse(alm0[:nchk].contiguous(), prec)
-            outs[prec] = (r.alm[:nchk], inv_of_ref)
-        result["accuracy_vs_fp64"] = accuracy(cpu, f[:nchk], alm0[:nchk], outs)
+        # the last timed step of each run left alm = forward(f) and
+        # out = inverse(alm) for all 64 fields
+        outs = {prec: (r.alm, r.out) for prec, (_, _, r) in runs.items()}
+        acc = accuracy(cpu, f, outs)
+        result["accuracy_vs_fp64"] = acc
+        # the metric is quoted at rel-L2 <= 2e-5: a run outside it is not a valid measurement
+        result["accuracy_ok"] = bool(acc[args.precision]["within_tolerance"])
     print(json.dumps(result))
 
 

@@ -7,6 +7,7 @@ pointers and streams therefore come straight from torch tensors and
 is no fallback path.
 """
 
+import atexit
 import ctypes
 import os
 import threading
@@ -32,6 +33,17 @@ REMAP_KINDS = {
 }
 
 PRECISIONS = {"fp32": 0, "bf16x3": 1, "bf16": 2}
+NORM_ORTHO, NORM_FOUR_PI, NORM_SCHMIDT = 0, 1, 2
+
+
+def norm_code(norm):
+    """The reference's norm strings (cuhpx/sht_tools.py:196-221): "ortho",
+    "schmidt", and every other value scaled as "four-pi"."""
+    if norm == "ortho":
+        return NORM_ORTHO
+    if norm == "schmidt":
+        return NORM_SCHMIDT
+    return NORM_FOUR_PI
 ADJOINT_MODES = {"exact": 0, "compat": 1}
 
 _lock = threading.Lock()
@@ -61,7 +73,8 @@ SIGNATURES = {
     "hpx_sht_inverse": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_forward_adjoint": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_inverse_adjoint": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
-    "hpx_vsht_plan_create_healpix": (_c_int, [_c_int, _c_int, _c_int, _dp, _c_int, _vp, ctypes.POINTER(_vp)]),
+    "hpx_vsht_plan_create_healpix": (_c_int, [_c_int, _c_int, _c_int, _dp, _c_int, _c_int, _c_int, _vp,
+                                              ctypes.POINTER(_vp)]),
     "hpx_vsht_workspace_bytes": (_c_i64, [_vp, _c_i64]),
     "hpx_vsht_forward": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_vsht_inverse": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
@@ -102,6 +115,24 @@ def lib():
                     fn.argtypes = args
                 _lib = handle
     return _lib
+
+
+_exiting = False
+
+
+@atexit.register
+def _mark_exit():
+    # Registered after torch's own handlers, so it runs before them.  From
+    # here on the finalisers of plans and mixed requests make no native
+    # calls: the process is ending and the driver reclaims device memory,
+    # while the HIP runtime may already be tearing down.
+    global _exiting
+    _exiting = True
+
+
+def exiting():
+    """True once the interpreter has started to exit."""
+    return _exiting
 
 
 def last_error():

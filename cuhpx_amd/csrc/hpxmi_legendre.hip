@@ -182,9 +182,13 @@ struct ScaledOrder {
 //           + sqrt((l+m+1)(l+m+2)) P^{m+1}_{l+1})  (m >= 1), = m P^m_l / sin;
 // both negated for odd m when csphase (the Condon-Shortley sign).  Only the
 // orders m - 1 and m + 1 are needed; each runs its own scaled recurrence.
+// norm (sht_tools.py:194-221, applied to the P rows the two tables are built
+// from): HPX_NORM_ORTHO leaves them; any other scales both by sqrt(4 pi),
+// and HPX_NORM_SCHMIDT by (2 L + 1)^(-1/2) at the degree L the row reads
+// (L = l for D0, l + 1 for D1); inverse tables take the reciprocals.
 __global__ __launch_bounds__(kThreads) void vector_table_kernel(
     float* __restrict__ D, int64_t vstride, const int64_t* __restrict__ p_off, const int* __restrict__ nl,
-    const double* __restrict__ ct, int npair, int kpad, int lmax, int csphase) {
+    const double* __restrict__ ct, int npair, int kpad, int lmax, int csphase, int norm, int inverse) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
   const int m = blockIdx.y;
   if (k >= kpad) return;
@@ -224,6 +228,16 @@ __global__ __launch_bounds__(kThreads) void vector_table_kernel(
       lo.advance(x);
     }
     if (l > m) hi.advance(x);  // keep degrees l + 1, l + 2 for the next l
+    if (norm != HPX_NORM_ORTHO) {
+      const double c = 3.5449077018110320546;  // sqrt(4 pi)
+      double f0 = c, f1 = c;
+      if (norm == HPX_NORM_SCHMIDT) {
+        f0 /= sqrt(2.0 * dl + 1.0);
+        f1 /= sqrt(2.0 * dl + 3.0);
+      }
+      d0 = inverse ? d0 / f0 : d0 * f0;
+      d1 = inverse ? d1 / f1 : d1 * f1;
+    }
     const int par = (l - m) & 1;
     const int64_t off = (int64_t)((l - m) >> 1) * kpad + k;
     T0[par][off] = sign * (float)d0;
@@ -908,7 +922,7 @@ int launch_legendre_table(ShtPlan& p, const double* d_ct, hipStream_t s) {
   dim3 grid((p.kpad + kThreads - 1) / kThreads, p.mmax);
   if (p.vector) {
     vector_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.vstride, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
-                                                  p.lmax, p.csphase);
+                                                  p.lmax, p.csphase, p.vnorm, p.vinverse);
   } else {
     legendre_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
                                                     p.lmax);

@@ -103,7 +103,7 @@ void free_plan(ShtPlan* p) {
 // vector: build the two tables of the vector transform (D0, D1) instead of P.
 int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const double* phi0,
                const double* weights, int lmax, int mmax, hipStream_t stream, ShtPlan** out,
-               int vector = 0, int csphase = 1) {
+               int vector = 0, int csphase = 1, int vnorm = 0, int vinverse = 0) {
   if (nring < 1) HPX_FAIL(HPX_ERR_ARG, "nring must be >= 1");
   if (lmax < 0 || mmax < 1) HPX_FAIL(HPX_ERR_ARG, "lmax must be >= 0 and mmax >= 1 (got %d, %d)", lmax, mmax);
   if (mmax > 65535 || lmax > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "lmax/mmax too large");
@@ -125,6 +125,8 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   p->mmax = mmax;
   p->vector = vector ? 1 : 0;
   p->csphase = csphase ? 1 : 0;
+  p->vnorm = vnorm;
+  p->vinverse = vinverse ? 1 : 0;
 
   std::vector<int64_t> offset(nring + 1, 0);
   for (int t = 0; t < nring; ++t) offset[t + 1] = offset[t] + nphi[t];
@@ -283,6 +285,17 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   p->at_rows = pfloats / p->kpad;
   p->vstride = p->vector ? pfloats : 0;
   const int64_t tfloats = std::max<int64_t>((p->vector ? 2 : 1) * pfloats, 1);
+  {
+    // the tables are the plan's only large allocation: refuse it with the
+    // sizes named rather than fail inside hipMalloc
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
+        (uint64_t)(2 * tfloats * sizeof(float)) > (uint64_t)free_b) {
+      free_plan(p);
+      HPX_FAIL(HPX_ERR_UNSUPPORTED, "plan tables need %.2f GB, %.2f GB free of %.2f GB on the device",
+               2.0 * tfloats * sizeof(float) / 1e9, free_b / 1e9, total_b / 1e9);
+    }
+  }
   double* d_ct = nullptr;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_p), tfloats * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->d_pt), tfloats * sizeof(float));
@@ -360,7 +373,7 @@ extern "C" int hpx_sht_plan_create_rings(int nring, const int32_t* nphi, const d
 
 namespace {
 int healpix_plan(int nside, int lmax, int mmax, const double* ring_weights, hipStream_t stream,
-                 hpx_sht_plan** plan, int vector, int csphase) {
+                 hpx_sht_plan** plan, int vector, int csphase, int vnorm = 0, int vinverse = 0) {
   if (plan == nullptr || ring_weights == nullptr) HPX_FAIL(HPX_ERR_ARG, "null argument");
   if (nside < 1) HPX_FAIL(HPX_ERR_ARG, "nside must be >= 1");
   if (nside > 2048) HPX_FAIL(HPX_ERR_UNSUPPORTED, "nside > 2048 exceeds the FFT size limit");
@@ -389,7 +402,7 @@ int healpix_plan(int nside, int lmax, int mmax, const double* ring_weights, hipS
   }
   ShtPlan* p = nullptr;
   const int rc = build_plan(nring, nphi.data(), z.data(), phi0.data(), ring_weights, lmax, mmax, stream, &p,
-                            vector, csphase);
+                            vector, csphase, vnorm, vinverse);
   if (rc != HPX_OK) return rc;
   *plan = H(p);
   return HPX_OK;
@@ -402,9 +415,13 @@ extern "C" int hpx_sht_plan_create_healpix(int nside, int lmax, int mmax, const 
 }
 
 extern "C" int hpx_vsht_plan_create_healpix(int nside, int lmax, int mmax, const double* ring_weights,
-                                            int csphase, hipStream_t stream, hpx_sht_plan** plan) {
+                                            int csphase, int norm, int inverse, hipStream_t stream,
+                                            hpx_sht_plan** plan) {
   if (lmax < 1) HPX_FAIL(HPX_ERR_ARG, "a vector plan needs lmax >= 1");
-  return healpix_plan(nside, lmax, mmax, ring_weights, stream, plan, 1, csphase);
+  if (norm != HPX_NORM_ORTHO && norm != HPX_NORM_FOUR_PI && norm != HPX_NORM_SCHMIDT)
+    HPX_FAIL(HPX_ERR_ARG, "unknown norm %d", norm);
+  return healpix_plan(nside, lmax, mmax, ring_weights, stream, plan, 1, csphase, norm,
+                      norm == HPX_NORM_ORTHO ? 0 : inverse);
 }
 
 extern "C" int64_t hpx_vsht_workspace_bytes(const hpx_sht_plan* plan, int64_t nbatch) {

@@ -87,6 +87,8 @@ struct ShtPlan {
   // (the same for d_pt), instead of P.
   int vector = 0;
   int csphase = 1;
+  int vnorm = 0;     // HPX_NORM_* of the vector tables
+  int vinverse = 0;  // vector tables scaled for the inverse transform
   int64_t vstride = 0;
 
   // Autotuned launch shapes (hpx_sht_autotune); untuned keys use defaults.

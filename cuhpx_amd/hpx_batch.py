@@ -114,7 +114,7 @@ class MixedSHT:
 
     def __del__(self):
         h = getattr(self, "_handle", None)
-        if h is not None and h.value:
+        if h is not None and h.value and not _lib.exiting():
             try:
                 _lib.lib().hpx_sht_mixed_destroy(h)
             except Exception:  # interpreter shutdown

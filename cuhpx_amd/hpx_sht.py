@@ -27,6 +27,7 @@ generation); this module only allocates outputs and the workspace.
 
 import ctypes
 import threading
+import weakref
 
 import numpy as np
 import torch
@@ -35,8 +36,21 @@ import torch.nn as nn
 from . import _lib
 from .sht_tools import W_helper, healpix_weights
 
+# Device plans shared by every module of the same grid, band limit and device.
+# The cache holds them weakly: a plan (and its device tables) is freed when
+# the last module, autograd context or caller holding it goes away.
 _plan_lock = threading.Lock()
-_plans = {}
+_plans = weakref.WeakValueDictionary()
+
+
+def _create(fn, *args):
+    """Run a plan-creation call; when the device lacks room for the tables,
+    return torch's cached free blocks to the device and try once more."""
+    rc = fn(*args)
+    if rc == _lib.HPX_ERR_UNSUPPORTED and "free" in _lib.last_error():
+        torch.cuda.empty_cache()
+        rc = fn(*args)
+    return rc
 
 
 class ShtPlan:
@@ -68,19 +82,25 @@ class ShtPlan:
                 w = np.ascontiguousarray(w, dtype=np.float64)
                 h = ctypes.c_void_p()
                 with torch.cuda.device(device):
-                    rc = _lib.lib().hpx_sht_plan_create_healpix(
-                        nside, lmax, mmax, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                        _lib.stream_ptr(device), ctypes.byref(h))
+                    rc = _create(_lib.lib().hpx_sht_plan_create_healpix,
+                                 nside, lmax, mmax, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 _lib.stream_ptr(device), ctypes.byref(h))
                 _lib.check(rc, "SHT plan")
                 plan = cls(h, device, 12 * nside * nside, 4 * nside - 1, lmax, mmax)
                 _plans[key] = plan
         return plan
 
     @classmethod
-    def healpix_vector(cls, nside, lmax, mmax, quad_weights="ring", csphase=True, device=None):
-        """Plan of the vector transform (VectorSHT / VectoriSHT tables)."""
+    def healpix_vector(cls, nside, lmax, mmax, quad_weights="ring", csphase=True, device=None, norm="ortho",
+                       inverse=False):
+        """Plan of the vector transform (VectorSHT / VectoriSHT tables).
+        ``norm`` scales the tables as the reference's legpoly does; with a
+        norm other than "ortho" the forward (VectorSHT) and inverse
+        (VectoriSHT, ``inverse=True``) tables differ and are separate plans."""
         device = torch.device("cuda", torch.cuda.current_device() if device is None else device.index)
-        key = ("healpix-vector", nside, lmax, mmax, quad_weights, bool(csphase), device.index)
+        code = _lib.norm_code(norm)
+        inv = bool(inverse) and code != _lib.NORM_ORTHO
+        key = ("healpix-vector", nside, lmax, mmax, quad_weights, bool(csphase), code, inv, device.index)
         with _plan_lock:
             plan = _plans.get(key)
             if plan is None:
@@ -88,9 +108,9 @@ class ShtPlan:
                 w = np.ascontiguousarray(w, dtype=np.float64)
                 h = ctypes.c_void_p()
                 with torch.cuda.device(device):
-                    rc = _lib.lib().hpx_vsht_plan_create_healpix(
-                        nside, lmax, mmax, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), int(bool(csphase)),
-                        _lib.stream_ptr(device), ctypes.byref(h))
+                    rc = _create(_lib.lib().hpx_vsht_plan_create_healpix,
+                                 nside, lmax, mmax, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                 int(bool(csphase)), code, int(inv), _lib.stream_ptr(device), ctypes.byref(h))
                 _lib.check(rc, "vector SHT plan")
                 plan = cls(h, device, 12 * nside * nside, 4 * nside - 1, lmax, mmax, vector=True)
                 _plans[key] = plan
@@ -110,9 +130,9 @@ class ShtPlan:
                 dptr = ctypes.POINTER(ctypes.c_double)
                 h = ctypes.c_void_p()
                 with torch.cuda.device(device):
-                    rc = _lib.lib().hpx_sht_plan_create_rings(
-                        len(nphi), nphi.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                        theta.ctypes.data_as(dptr), phi0.ctypes.data_as(dptr),
+                    rc = _create(
+                        _lib.lib().hpx_sht_plan_create_rings, len(nphi),
+                        nphi.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), theta.ctypes.data_as(dptr), phi0.ctypes.data_as(dptr),
                         weights.ctypes.data_as(dptr), lmax, mmax, _lib.stream_ptr(device),
                         ctypes.byref(h))
                 _lib.check(rc, "SHT plan")
@@ -123,7 +143,7 @@ class ShtPlan:
 
     def __del__(self):
         h = getattr(self, "_handle", None)
-        if h is not None and h.value:
+        if h is not None and h.value and not _lib.exiting():
             try:
                 _lib.lib().hpx_sht_plan_destroy(h)
             except Exception:  # interpreter shutdown
@@ -338,6 +358,8 @@ class iSHTCUDA(_HealpixSHTBase):
 
 
 class _HealpixVectorBase(nn.Module):
+    _inverse = False
+
     def __init__(self, nside, lmax, mmax, grid, quad_weights, norm, csphase, use_bluestein, precision):
         super().__init__()
         if grid != "healpix":
@@ -357,14 +379,15 @@ class _HealpixVectorBase(nn.Module):
         self.precision = precision
         if not torch.cuda.is_available():
             raise RuntimeError("CUDA device is not available. This class requires a GPU.")
-        self._plan = ShtPlan.healpix_vector(nside, self.lmax, self.mmax, quad_weights, csphase)
+        self._plan = ShtPlan.healpix_vector(nside, self.lmax, self.mmax, quad_weights, csphase, norm=norm,
+                                            inverse=self._inverse)
 
     def plan_for(self, device):
         if device.type != "cuda":
             raise RuntimeError("VectorSHT / VectoriSHT need a GPU tensor")
         if self._plan.device.index != device.index:
             self._plan = ShtPlan.healpix_vector(self.nside, self.lmax, self.mmax, self.quad_weights, self.csphase,
-                                                device)
+                                                device, norm=self.norm, inverse=self._inverse)
         return self._plan
 
 
@@ -374,8 +397,12 @@ class VectorSHT(_HealpixVectorBase):
     (spheroidal, toroidal coefficients).
 
     Same constructor as the reference (``grid`` must be "healpix";
-    ``norm`` and ``use_bluestein`` are accepted and, as there, have no
-    effect for the orthonormal HEALPix transform; ``csphase`` applies the
+    ``norm`` scales the tables as the reference's legpoly(norm=...) does,
+    sht_tools.py:196-221: "ortho" (default) leaves them, "schmidt" scales
+    the rows of degree L by sqrt(4 pi / (2 L + 1)) (L = l for the
+    theta-derivative, l + 1 for the m / sin(theta) table, as the reference
+    builds them), any other value by sqrt(4 pi); ``use_bluestein`` is
+    accepted and, as there, has no effect; ``csphase`` applies the
     Condon-Shortley sign to odd m, default True as in sht_tools.py:302-304),
     plus ``precision`` ("fp32", "bf16x3", "bf16") for the Legendre step.
     Like the reference, whose ring FFTs carry no autograd, the output does not
@@ -403,7 +430,11 @@ class VectoriSHT(_HealpixVectorBase):
     """Inverse vector SHT (cuhpx/hpx_sht.py:377-444): complex
     [..., 2, lmax, mmax] (spheroidal, toroidal) -> real [..., 2, 12 nside^2]
     (theta and phi components, RING).  Constructor as the reference's plus
-    ``precision``; the output does not track gradients."""
+    ``precision``; ``norm`` as in VectorSHT with the reciprocal factors
+    (legpoly(inverse=True), hpx_sht.py:396); the output does not track
+    gradients."""
+
+    _inverse = True
 
     def __init__(self, nside, lmax=None, mmax=None, grid="healpix", norm="ortho", csphase=True,
                  use_bluestein=False, precision="fp32"):

@@ -175,16 +175,28 @@ int hpx_sht_inverse_adjoint(const hpx_sht_plan* plan, int precision, int mode,
 /* Vector (spheroidal / toroidal) SHT                                  */
 /* ------------------------------------------------------------------ */
 
+/* Normalisations of the vector tables (the reference's norm argument,
+ * cuhpx/sht_tools.py:194-221): "ortho", "schmidt", and any other string,
+ * which the reference scales like HPX_NORM_FOUR_PI. */
+enum { HPX_NORM_ORTHO = 0, HPX_NORM_FOUR_PI = 1, HPX_NORM_SCHMIDT = 2 };
+
 /* Plan for the vector transform on a HEALPix grid: the state VectorSHT /
  * VectoriSHT build in their constructors (cuhpx/hpx_sht.py:288-330, 379-402):
  * ring geometry and weights as hpx_sht_plan_create_healpix, and instead of
  * the scalar table the two tables of _precompute_dlegpoly
  * (cuhpx/sht_tools.py:267-304), D0 = dP/dtheta and D1 = m P / sin(theta),
  * generated on the device in FP64 (Condon-Shortley sign on odd m when
- * csphase != 0, the reference's default).  The scalar entry points reject a
- * vector plan and the vector ones a scalar plan.  Synchronises the stream. */
+ * csphase != 0, the reference's default).  norm scales the P rows the tables
+ * are built from as legpoly(norm=..., inverse=...) does (sht_tools.py:
+ * 196-221): by sqrt(4 pi) unless HPX_NORM_ORTHO, and for HPX_NORM_SCHMIDT by
+ * (2 L + 1)^(-1/2) at the row's degree L (l for D0, l + 1 for D1); with
+ * inverse != 0 (VectoriSHT's tables, hpx_sht.py:396) by the reciprocals.
+ * With HPX_NORM_ORTHO the tables do not depend on inverse.  The scalar
+ * entry points reject a vector plan and the vector ones a scalar plan.
+ * Synchronises the stream. */
 int hpx_vsht_plan_create_healpix(int nside, int lmax, int mmax,
                                  const double* ring_weights, int csphase,
+                                 int norm, int inverse,
                                  hipStream_t stream, hpx_sht_plan** plan);
 
 /* Workspace (bytes) for nbatch vector fields (that of 2*nbatch scalar maps). */

@@ -157,6 +157,25 @@ class SHTOracle:
             out[..., :, m] = a[..., :, m] @ self.pct[m]
         return out
 
+    def forward_and_inverse(self, f, a):
+        """(forward(f), inverse(a)) with one sweep over the table: a streamed
+        oracle generates P once for both directions instead of twice."""
+        if not self.streamed:
+            return self.forward(f), self.inverse(a)
+        ftm = ring_rfft(f, self.grid, self.mmax) * self.grid.weight[:, None]
+        a = np.asarray(a, dtype=np.complex128)
+        lead_f, lead_a = ftm.shape[:-2], a.shape[:-2]
+        fm = np.moveaxis(ftm.reshape((-1,) + ftm.shape[-2:]), -1, 0)  # [m, F, k]
+        am = np.moveaxis(a.reshape((-1,) + a.shape[-2:]), -1, 0)      # [m, F, l]
+        alm = np.zeros((fm.shape[1], self.lmax, self.mmax), dtype=np.complex128)
+        acc = np.zeros((self.mmax, am.shape[1], self.grid.nring), dtype=np.complex128)
+        for l0, q in self._lblocks():
+            lb = q.shape[1]
+            alm[:, l0:l0 + lb, :] = np.moveaxis(np.matmul(fm, np.swapaxes(q, 1, 2)), 0, -1)
+            acc += np.matmul(am[:, :, l0:l0 + lb], q.astype(np.complex128))
+        g = np.moveaxis(acc, 0, -1).reshape(lead_a + (self.grid.nring, self.mmax))
+        return alm.reshape(lead_f + (self.lmax, self.mmax)), ring_irfft(g, self.grid, self.mmax, "irfft")
+
     def forward(self, f):
         """a[..., lmax, mmax] = sum_t w_t P F_t (hpx_sht.py:214-236)."""
         ftm = ring_rfft(f, self.grid, self.mmax)

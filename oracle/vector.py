@@ -34,10 +34,24 @@ from .legendre import legpoly
 from .sht import ring_irfft, ring_rfft
 
 
-def dlegpoly(mmax, lmax, x, csphase=True):
+def norm_scale(lmax, norm="ortho", inverse=False):
+    """Per-degree factor legpoly(norm=..., inverse=...) applies to its rows
+    (sht_tools.py:196-221): 1 for "ortho"; sqrt(4 pi) for any other norm,
+    times (2L+1)^(-1/2) for "schmidt"; reciprocals when ``inverse``."""
+    L = np.arange(lmax, dtype=np.float64)
+    f = np.ones(lmax)
+    if norm != "ortho":
+        f *= np.sqrt(4 * np.pi)
+        if norm == "schmidt":
+            f /= np.sqrt(2 * L + 1)
+    return 1.0 / f if inverse else f
+
+
+def dlegpoly(mmax, lmax, x, csphase=True, norm="ortho", inverse=False):
     """dpct[2, mmax, lmax, len(x)] of sht_tools.py:267-304, vectorised over m."""
     x = np.asarray(x, dtype=np.float64)
     pct = legpoly(mmax + 1, lmax + 1, x)  # P^m_l for m <= mmax, l <= lmax
+    pct = pct * norm_scale(lmax + 1, norm, inverse)[None, :, None]
     d = np.zeros((2, mmax, lmax, len(x)), dtype=np.float64)
     for l in range(lmax):
         # m = 0 (sht_tools.py:279)
@@ -80,17 +94,20 @@ def _contract_km(a, W):
 class VectorSHTOracle:
     """FP64 VectorSHT / VectoriSHT on a ring grid."""
 
-    def __init__(self, grid, lmax, mmax, csphase=True):
+    def __init__(self, grid, lmax, mmax, csphase=True, norm="ortho"):
         self.grid = grid
         self.lmax = lmax
         self.mmax = mmax
-        self.dpct = dlegpoly(mmax, lmax, np.cos(grid.theta), csphase)
+        x = np.cos(grid.theta)
+        self.dpct = dlegpoly(mmax, lmax, x, csphase, norm)  # VectorSHT's table
+        # VectoriSHT's (hpx_sht.py:396: inverse=True); the same for "ortho"
+        self.dpct_inv = self.dpct if norm == "ortho" else dlegpoly(mmax, lmax, x, csphase, norm, inverse=True)
 
     @classmethod
-    def healpix(cls, nside, lmax=None, mmax=None, quad_weights="ring", csphase=True):
+    def healpix(cls, nside, lmax=None, mmax=None, quad_weights="ring", csphase=True, norm="ortho"):
         lmax = lmax or 4 * nside - 1
         mmax = mmax or 2 * nside + 1
-        return cls(RingGrid.healpix(nside, quad_weights), lmax, mmax, csphase)
+        return cls(RingGrid.healpix(nside, quad_weights), lmax, mmax, csphase, norm)
 
     def norm_factor(self):
         """1 / (l (l + 1)) with 1 at l = 0 (hpx_sht.py:324-326)."""
@@ -114,7 +131,7 @@ class VectorSHTOracle:
         """(S, T) coefficients [..., 2, lmax, mmax] -> v[..., 2, npix]."""
         c = np.asarray(c, dtype=np.complex128)
         a0, a1 = c[..., 0, :, :], c[..., 1, :, :]
-        D0, D1 = self.dpct[0], self.dpct[1]
+        D0, D1 = self.dpct_inv[0], self.dpct_inv[1]
         s = _contract_km(a0, D0) + 1j * _contract_km(a1, D1)
         t = 1j * _contract_km(a0, D1) - _contract_km(a1, D0)
         return ring_irfft(np.stack([s, t], axis=-3), self.grid, self.mmax, "irfft")

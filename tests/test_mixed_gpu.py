@@ -79,6 +79,48 @@ def test_mixed_graph_matches_per_size_calls_and_oracle():
         assert err_a < 2e-5 and err_f < 2e-5
 
 
+C4_NSIDES = (64, 128, 256, 512)
+C4_FIELDS = 32
+
+
+@pytest.mark.timeout(1200)
+def test_mixed_request_at_full_sizes():
+    """Config: 32 fields each at nside 64, 128, 256 and 512, lmax = mmax =
+    2 nside + 1, split-BF16, autotuned, forward then inverse in one graph
+    launch.  Inputs are band-limited (a_lm with C_l = (l+1)^-2, seed
+    10 + log2 nside, synthesised by the FP32 inverse).  Every size's outputs
+    equal the per-size SHTCUDA / iSHTCUDA calls bit for bit, and the first and
+    the last field of each size match the streamed FP64 oracle within the
+    split-BF16 tolerance (forward on the FP32 input, inverse on the graph's
+    own coefficients)."""
+    specs = [(n, C4_FIELDS, 2 * n + 1, 2 * n + 1) for n in C4_NSIDES]
+    print(f"[mixed C4] building {specs}", flush=True)
+    mixed = cuhpx_amd.MixedSHT(specs, precision="bf16x3", direction="both", autotune=True)
+    for i, (n, b, L, _) in enumerate(specs):
+        a = band_limited_alm(L, L, b, 10 + int(np.log2(n))).astype(np.complex64)
+        mixed.maps_in[i].copy_(cuhpx_amd.iSHTCUDA(n, lmax=L, mmax=L)(torch.from_numpy(a).cuda()))
+    mixed.launch()
+    torch.cuda.synchronize()
+    alm = [a.clone() for a in mixed.alm]
+    back = [g.clone() for g in mixed.maps_out]
+    chk = [0, C4_FIELDS - 1]
+    for i, (n, b, L, _) in enumerate(specs):
+        x = mixed.maps_in[i]
+        sht = cuhpx_amd.SHTCUDA(n, lmax=L, mmax=L, precision="bf16x3")
+        isht = cuhpx_amd.iSHTCUDA(n, lmax=L, mmax=L, precision="bf16x3")
+        assert torch.equal(alm[i], sht(x)), f"nside {n} forward differs from the per-size call"
+        assert torch.equal(back[i], isht(alm[i])), f"nside {n} inverse differs from the per-size call"
+        o = S.SHTOracle.healpix(n, L, L, streamed=True)
+        a_ref, g_ref = o.forward_and_inverse(x[chk].double().cpu().numpy(),
+                                             alm[i][chk].cpu().numpy().astype(np.complex128))
+        a_got = alm[i][chk].cpu().numpy()
+        g_got = back[i][chk].cpu().numpy()
+        for j, fld in enumerate(chk):
+            ea, ef = rel_l2(a_got[j], a_ref[j]), rel_l2(g_got[j], g_ref[j])
+            print(f"[mixed C4 nside={n} field {fld}] forward {ea:.2e} inverse {ef:.2e}", flush=True)
+            assert ea < 2e-5 and ef < 2e-5, (n, fld, ea, ef)
+
+
 def test_mixed_graph_replays_match_eager_results():
     """Several replays with eager work and synchronisation in between; every
     replay reproduces the first."""
@@ -123,3 +165,22 @@ def test_mixed_rejects_bad_requests():
         cuhpx_amd.MixedSHT([(8, 2)], precision="fp16")
     with pytest.raises(ValueError):
         cuhpx_amd.MixedSHT([(8, 0)])
+
+
+def test_process_exit_with_live_graph_and_plans():
+    """A process that ends with a launched mixed request and its plans still
+    alive (no synchronisation, no explicit release) exits cleanly: after the
+    interpreter starts to exit, the finalisers make no native calls."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import torch, cuhpx_amd\n"
+            "m = cuhpx_amd.MixedSHT([(8, 2), (16, 3)], precision='bf16x3', direction='both', autotune=False)\n"
+            "m.launch()\n"
+            "sht = cuhpx_amd.SHTCUDA(16)\n"
+            "a = sht(torch.randn(3, 12 * 16 * 16, device='cuda'))\n"
+            "print('launched', flush=True)\n")
+    res = subprocess.run([sys.executable, "-c", code], cwd=root, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr[-2000:]
+    assert "launched" in res.stdout

@@ -299,3 +299,14 @@ def test_l_outer_scaled_recurrence_equals_per_m_recurrence():
             assert q.shape[0] == min(l + 1, mmax)
             got[:q.shape[0], l, :] = q
         assert np.array_equal(got, ref), (mmax, lmax)
+
+
+def test_streamed_forward_and_inverse_share_one_sweep():
+    """forward_and_inverse equals forward and inverse called one by one."""
+    o = S.SHTOracle.healpix(8, 17, 17, streamed=True)
+    rng = np.random.default_rng(12)
+    f = rng.standard_normal((2, o.grid.npix))
+    a = rng.standard_normal((3, 17, 17)) + 1j * rng.standard_normal((3, 17, 17))
+    fa, ga = o.forward_and_inverse(f, a)
+    np.testing.assert_allclose(fa, o.forward(f), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(ga, o.inverse(a), rtol=0, atol=1e-13)

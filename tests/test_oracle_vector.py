@@ -94,3 +94,36 @@ def test_forward_inverts_inverse_on_band_limited_fields():
     only_s[1] = 0
     leak = o.forward(o.inverse(only_s))[1]
     assert np.linalg.norm(leak) < 1e-4 * np.linalg.norm(c[0])
+
+
+@pytest.mark.parametrize("inverse", [False, True])
+def test_norms_against_scipy_unnormalised_legendre(inverse):
+    """norm="schmidt" rows are built from sqrt((l-m)!/(l+m)!) P^m_l (P from
+    scipy's lpmv, which carries the CS sign): the theta-derivative of that
+    function at degree l, and m / sin(theta) times it with the degree-(l+1)
+    factor sqrt((2l+1)/(2l+3)) the reference's ladder identity leaves; any
+    other norm but "ortho" is sqrt(4 pi) times the orthonormal rows; inverse
+    tables take the reciprocal factors."""
+    from math import factorial
+    import scipy.special as sp
+    th = _thetas()
+    h = 1e-6
+    lmax = mmax = 9
+    d_s = dlegpoly(mmax, lmax, np.cos(th), True, "schmidt", inverse)
+    d_o = dlegpoly(mmax, lmax, np.cos(th), True, "ortho")
+    d_4 = dlegpoly(mmax, lmax, np.cos(th), True, "four-pi", inverse)
+    c4 = np.sqrt(4 * np.pi)
+    np.testing.assert_allclose(d_4, d_o / c4 if inverse else d_o * c4, rtol=1e-13, atol=0)
+    for l in range(lmax):
+        for m in range(min(l + 1, mmax)):
+            q = np.sqrt(factorial(l - m) / factorial(l + m))
+            fs = (lambda t: q * sp.lpmv(m, l, np.cos(t)))
+            d0 = (fs(th + h) - fs(th - h)) / (2 * h)
+            d1 = np.sqrt((2 * l + 1) / (2 * l + 3)) * m * fs(th) / np.sin(th)
+            if inverse:  # schmidt inverse = schmidt forward * (2L+1) / (4 pi) at the row's degree
+                d0 = d0 * (2 * l + 1) / (4 * np.pi)
+                d1 = d1 * (2 * l + 3) / (4 * np.pi)
+            scale = max(np.max(np.abs(d0)), 1e-30)
+            np.testing.assert_allclose(d_s[0, m, l], d0, rtol=0, atol=1e-7 * scale, err_msg=f"D0 l={l} m={m}")
+            np.testing.assert_allclose(d_s[1, m, l], d1, rtol=0, atol=1e-12 * max(np.max(np.abs(d1)), 1.0),
+                                       err_msg=f"D1 l={l} m={m}")

@@ -147,6 +147,26 @@ def test_scalar_and_vector_plans_reject_each_others_calls():
     assert rc == 1 and b"workspace" in lib.hpx_last_error()
     h = ctypes.c_void_p()
     w = np.ones(4 * nside - 1)
-    rc = lib.hpx_vsht_plan_create_healpix(nside, 0, 9, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1, s,
+    rc = lib.hpx_vsht_plan_create_healpix(nside, 0, 9, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1, 0, 0, s,
                                           ctypes.byref(h))
     assert rc == 1
+
+
+@pytest.mark.parametrize("norm", ["schmidt", "four-pi"])
+def test_vector_norms_match_oracle(norm):
+    """norm scales the tables as the reference's legpoly does (VectorSHT with
+    the forward factors, VectoriSHT with their reciprocals)."""
+    nside, lmax, mmax = 8, 17, 13
+    o = VectorSHTOracle.healpix(nside, lmax, mmax, norm=norm)
+    rng = np.random.default_rng(33)
+    f = rng.standard_normal((2, 2, o.grid.npix)).astype(np.float32)
+    fwd = cuhpx_amd.VectorSHT(nside, lmax=lmax, mmax=mmax, norm=norm)
+    inv = cuhpx_amd.VectoriSHT(nside, lmax=lmax, mmax=mmax, norm=norm)
+    c = fwd(torch.from_numpy(f).cuda()).cpu().numpy()
+    assert rel_l2(c, o.forward(f.astype(np.float64))) < 1e-5
+    a = vector_coeffs(lmax, mmax, 2, 34).astype(np.complex64)
+    g = inv(torch.from_numpy(a).cuda()).cpu().numpy()
+    assert rel_l2(g, o.inverse(a.astype(np.complex128))) < 1e-5
+    # and the scaling is not the identity
+    ortho = cuhpx_amd.VectorSHT(nside, lmax=lmax, mmax=mmax)(torch.from_numpy(f).cuda()).cpu().numpy()
+    assert rel_l2(c, ortho) > 0.5
