@@ -59,69 +59,80 @@ __device__ __forceinline__ void split(float x, __bf16& hi, __bf16& lo) {
 // ---------------------------------------------------------------------------
 
 // One thread per (m, k): P_mm by the sectoral product, then the three-term
-// recurrence in l (cuhpx/sht_tools.py:243-252), each value carried as
-// v * 2^(600 e); a value is stored when e == 0 and as 0 while e < 0
-// (|P| < 2^-600, far below FP32's range).
-__global__ __launch_bounds__(kThreads) void legendre_table_kernel(
-    float* __restrict__ P, const int64_t* __restrict__ p_off, const int* __restrict__ nl,
-    const double* __restrict__ ct, int npair, int kpad, int lmax) {
+// recurrence in l (cuhpx/sht_tools.py:243-252), in FP64, each value carried
+// as v * 2^(600 e) (a value is zero while e < 0: |P| < 2^-600, far below
+// FP32's range).  At the first degree l0 = m + 16 s of every segment s the
+// pair (P_{l0}, P_{l0+1}) is stored, rounded once to FP32; the GEMMs
+// regenerate the segment's other 14 degrees from it in FP32.
+__global__ __launch_bounds__(kThreads) void checkpoint_kernel(float2* __restrict__ ck,
+                                                              const int* __restrict__ seg_off,
+                                                              const double* __restrict__ ct, int npair,
+                                                              int kpad, int lmax) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
   const int m = blockIdx.y;
   if (k >= kpad) return;
-  const int nl0 = nl[2 * m], nl1 = nl[2 * m + 1];
-  const int pad0 = (nl0 + kRowPad - 1) / kRowPad * kRowPad;
-  const int pad1 = (nl1 + kRowPad - 1) / kRowPad * kRowPad;
-  float* P0 = P + p_off[m];
-  float* P1 = P0 + (int64_t)pad0 * kpad;
-  // padded rows and padded ring pairs are zero
-  for (int i = nl0; i < pad0; ++i) P0[(int64_t)i * kpad + k] = 0.f;
-  for (int i = nl1; i < pad1; ++i) P1[(int64_t)i * kpad + k] = 0.f;
+  const int ns = seg_off[m + 1] - seg_off[m];
+  float2* C = ck + (int64_t)seg_off[m] * kpad + k;
   if (k >= npair) {
-    for (int i = 0; i < nl0; ++i) P0[(int64_t)i * kpad + k] = 0.f;
-    for (int i = 0; i < nl1; ++i) P1[(int64_t)i * kpad + k] = 0.f;
+    for (int t = 0; t < ns; ++t) C[(int64_t)t * kpad] = make_float2(0.f, 0.f);
     return;
   }
-  if (m >= lmax) return;
+  if (ns == 0) return;
   const double x = ct[k];
-  const double s = sqrt((1.0 - x) * (1.0 + x));
+  const double sn = sqrt((1.0 - x) * (1.0 + x));
   const double big = 0x1p600, tiny = 0x1p-600;
   double pmm = 0.28209479177387814347;  // 1 / sqrt(4 pi)
   int e = 0;
   for (int i = 1; i <= m; ++i) {
-    pmm *= sqrt((2.0 * i + 1.0) / (2.0 * i)) * s;
+    pmm *= sqrt((2.0 * i + 1.0) / (2.0 * i)) * sn;
     if (pmm != 0.0 && fabs(pmm) < tiny) {
       pmm *= big;
       --e;
     }
   }
-  // l = m
-  P0[k] = e == 0 ? (float)pmm : 0.f;
-  if (m + 1 >= lmax) return;
-  // l = m + 1
-  double p2 = pmm;                              // P_{l-2}
-  double p1 = sqrt(2.0 * m + 3.0) * x * pmm;    // P_{l-1}
+  // invariant at the top of the loop: p0 = P_l, p1 = P_{l+1} (both scaled by 2^(600 e))
+  double p0 = pmm;
+  double p1 = m + 1 < lmax ? sqrt(2.0 * m + 3.0) * x * pmm : 0.0;
   if (e < 0 && fabs(p1) > 1.0) {
+    p0 *= tiny;
     p1 *= tiny;
-    p2 *= tiny;
     ++e;
   }
-  P1[k] = e == 0 ? (float)p1 : 0.f;
   const double dm = m;
-  for (int l = m + 2; l < lmax; ++l) {
-    const double dl = l;
-    const double a = sqrt((2 * dl - 1) / (dl - dm) * (2 * dl + 1) / (dl + dm));
-    const double b =
-        sqrt((dl + dm - 1) / (dl - dm) * (2 * dl + 1) / (2 * dl - 3) * (dl - dm - 1) / (dl + dm));
-    const double v = x * a * p1 - b * p2;
-    p2 = p1;
+  for (int l = m; l < lmax; ++l) {
+    if (((l - m) & (kSegL - 1)) == 0)
+      C[(int64_t)((l - m) / kSegL) * kpad] =
+          e == 0 ? make_float2((float)p0, l + 1 < lmax ? (float)p1 : 0.f) : make_float2(0.f, 0.f);
+    double v = 0.0;
+    if (l + 2 < lmax) {
+      const double dl = l + 2;
+      const double a = sqrt((2 * dl - 1) / (dl - dm) * (2 * dl + 1) / (dl + dm));
+      const double b =
+          sqrt((dl + dm - 1) / (dl - dm) * (2 * dl + 1) / (2 * dl - 3) * (dl - dm - 1) / (dl + dm));
+      v = x * a * p1 - b * p0;
+    }
+    p0 = p1;
     p1 = v;
     if (e < 0 && fabs(p1) > 1.0) {
+      p0 *= tiny;
       p1 *= tiny;
-      p2 *= tiny;
       ++e;
     }
-    float* dst = ((l - m) & 1) ? P1 : P0;
-    dst[(int64_t)((l - m) >> 1) * kpad + k] = e == 0 ? (float)p1 : 0.f;
+  }
+}
+
+// One segment of P_lm for one ring pair (cos theta = x): v[j] = P_{l0 + j},
+// j < 16, from the stored pair v[0], v[1] by
+// P_l = a_l x P_{l-1} - b_l P_{l-2} in FP32 (ab[j] = (a, b) of l0 + j; zero
+// from lmax on, so the degrees past the band limit come out zero).
+template <class AB>
+__device__ __forceinline__ void segment_recurrence(float (&v)[kSegL], float2 c, float x, AB ab) {
+  v[0] = c.x;
+  v[1] = c.y;
+#pragma unroll
+  for (int j = 2; j < kSegL; ++j) {
+    const float2 q = ab(j);
+    v[j] = fmaf(q.x * x, v[j - 1], -(q.y * v[j - 2]));
   }
 }
 
@@ -443,11 +454,25 @@ __device__ __forceinline__ float vec_sign1(int lane, bool forward) {
   return ((lane & 1) != 0) == forward ? -1.f : 1.f;
 }
 
-// VEC: the vector transform; the reduction is 2 kpad long (D0 half, then D1
-// half at P + vstride), see vec_sign0 / vec_sign1.
+// What the scalar GEMMs generate P_lm from (the plan's segment checkpoints,
+// recurrence coefficients and ring cosines).
+struct OtfArgs {
+  const float2* __restrict__ ck;
+  const float2* __restrict__ ab;
+  const int* __restrict__ seg_off;
+  const float* __restrict__ ct;
+};
+
+// Scalar transform (VEC = false): the table tile of each k chunk is
+// generated in the work-group: thread (segment s, ring pair k) runs the FP32
+// recurrence of its 16 degrees (8 rows of each parity) from the segment's
+// checkpoint and writes them into the tile's column k.
+// VEC: the vector transform, tiles read from the stored tables; the
+// reduction is 2 kpad long (D0 half, then D1 half at P + vstride), see
+// vec_sign0 / vec_sign1.
 template <int PREC, int CT, bool VEC = false>
 __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_fwd_kernel(
-    const float* __restrict__ X, float* __restrict__ aT, const float* __restrict__ P,
+    const float* __restrict__ X, float* __restrict__ aT, const float* __restrict__ P, OtfArgs g,
     const int64_t* __restrict__ p_off, const int* __restrict__ nl, const int* __restrict__ kmin,
     const int2* __restrict__ items, int nitems, int ncb, int kpad, int64_t ncol, int64_t vstride) {
   // table tiles (double buffer) during the main loop, the output tile after it
@@ -491,6 +516,22 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
   const int srow = row0 + sr;
   const bool srow_ok = srow < (sp ? padp1 : padp0);
   const float* Psrc = Pm + (sp ? (int64_t)padp0 * kpad : 0) + (int64_t)srow * kpad + sh;
+  // generation (scalar): 8 segments (this work-group's 128 degrees) x 32 ring
+  // pairs; the segments' recurrence coefficients stay in LDS
+  __shared__ float2 abl[VEC ? 1 : 8][kSegL];
+  const int gs = threadIdx.x >> 5;
+  const int gk = threadIdx.x & 31;
+  const int sbase = VEC ? 0 : g.seg_off[m];
+  const int nsg = VEC ? 0 : g.seg_off[m + 1] - sbase;
+  const bool gseg_ok = jb * 8 + gs < nsg;
+  const float2* ckp = VEC ? nullptr : g.ck + (int64_t)(sbase + jb * 8 + gs) * kpad + gk;
+  if constexpr (!VEC) {
+    if (threadIdx.x < 8 * kSegL) {
+      const int ts = threadIdx.x / kSegL, tj = threadIdx.x % kSegL;
+      abl[ts][tj] = jb * 8 + ts < nsg ? g.ab[(int64_t)(sbase + jb * 8 + ts) * kSegL + tj] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+  }
   // B fragments from X[m] (both parities): the rows beyond the last ring
   // pair are zero, so no k bound check
   const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + (int64_t)m * 2 * pstride, 2 * pstride * 4);
@@ -499,6 +540,13 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
   const float sgn0 = vec_sign0(lane), sgn1 = vec_sign1(lane, true);
 
   auto load_a = [&](int kc, float* v) {
+    if constexpr (!VEC) {  // the checkpoint and the ring cosine; store_a generates
+      const float2 c = gseg_ok ? ckp[kc] : make_float2(0.f, 0.f);
+      v[0] = c.x;
+      v[1] = c.y;
+      v[2] = g.ct[kc + gk];
+      return;
+    }
     const float* src = Psrc + kc;
     if (VEC && kc >= kpad) src += vstride - kpad;
     if (srow_ok) {
@@ -516,6 +564,22 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
     }
   };
   auto store_a = [&](FwdLds<PREC>& L, const float* v) {
+    if constexpr (!VEC) {
+      float w[kSegL];
+      segment_recurrence(w, make_float2(v[0], v[1]), v[2], [&](int j) { return abl[gs][j]; });
+#pragma unroll
+      for (int j = 0; j < kSegL; ++j) {
+        const int p = j & 1, r = 8 * gs + (j >> 1);
+        if constexpr (PREC == HPX_PREC_FP32) {
+          L.a[p][r][gk] = w[j];
+        } else if constexpr (PREC == HPX_PREC_BF16X3) {
+          split(w[j], L.hi[p][r][bf16_slot(r, gk)], L.lo[p][r][bf16_slot(r, gk)]);
+        } else {
+          L.hi[p][r][bf16_slot(r, gk)] = (__bf16)w[j];
+        }
+      }
+      return;
+    }
     if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) L.a[sp][sr][sh + q] = v[q];
@@ -608,9 +672,14 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
 // VEC: the vector transform; the reduction runs over the rows of both
 // parities (D0T half against the opposite parity's coefficients, then the
 // D1T half at PT + vstride against the same parity's), see vec_sign0.
+// Scalar transform (VEC = false): the table tile of each i chunk is generated
+// in the work-group: thread (ring pair k, segment g of the chunk's four) runs
+// the FP32 recurrence of the segment's 16 degrees, which are 8 consecutive
+// rows i of each parity, so they land in the tile as one 16-byte row piece
+// per parity.
 template <int PREC, int CT, int RH, bool VEC = false>
 __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_kernel(
-    const float* __restrict__ aT, float* __restrict__ X, const float* __restrict__ PT,
+    const float* __restrict__ aT, float* __restrict__ X, const float* __restrict__ PT, OtfArgs g,
     const int64_t* __restrict__ p_off, const int* __restrict__ nl, const int* __restrict__ kmin, int nkb,
     int nitems, int ncb, int kpad, int64_t ncol, int64_t at_rows, int64_t vstride) {
   // same tiling as the forward kernel with the roles of i and k exchanged:
@@ -674,9 +743,24 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
   const int bvoff3 = (frag_k_lane<PREC>(lane) * nc + col0 + ((lane & 15) ^ 3)) * 4;
   const float sgn0 = vec_sign0(lane), sgn1 = vec_sign1(lane, false);
   const int kp = (nmax + kKTile - 1) / kKTile * kKTile;  // length of one half
+  // generation (scalar): ring pair gk of the block, segment gg of the chunk
+  // (wave-uniform, so the recurrence coefficients are scalar loads)
+  const int gk = threadIdx.x & (kRows - 1);
+  const int gg = __builtin_amdgcn_readfirstlane(threadIdx.x / kRows);
+  const int gkq = k0 + gk;
+  const float gx = (!VEC && gkq < kpad) ? g.ct[gkq] : 0.f;
+  const int sbase = VEC ? 0 : g.seg_off[m];
+  const int nsg = VEC ? 0 : g.seg_off[m + 1] - sbase;
 
   // table tile: this block's 64 ring pairs, i chunk starting at ic
   auto load_a = [&](int ic, float* v) {
+    if constexpr (!VEC) {  // the checkpoint; store_a generates
+      const int seg = ic / 8 + gg;
+      const float2 c = (seg < nsg && gkq < kpad) ? g.ck[(int64_t)(sbase + seg) * kpad + gkq] : make_float2(0.f, 0.f);
+      v[0] = c.x;
+      v[1] = c.y;
+      return;
+    }
     const bool half1 = VEC && ic >= kp;
     const int ii = half1 ? ic - kp : ic;
     const int spad = half1 ? spad1 : spad0;
@@ -695,7 +779,41 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
       for (int q = 0; q < 16; ++q) v[q] = 0.f;
     }
   };
-  auto store_a = [&](Lds& L, const float* v) {
+  // ic: the chunk of the tile being written (generation only)
+  auto store_a = [&](Lds& L, const float* v, int ic = 0) {
+    if constexpr (!VEC) {
+      // coefficients of a segment past the last one are never used (zero
+      // checkpoint); read the last segment's instead of running off the table
+      const int seg = min(ic / 8 + gg, nsg - 1);
+      const float2* abp = g.ab + (int64_t)(sbase + seg) * kSegL;
+      float w[kSegL];
+      segment_recurrence(w, make_float2(v[0], v[1]), gx, [&](int j) { return abp[j]; });
+      const int c8 = 8 * gg;
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        if constexpr (PREC == HPX_PREC_FP32) {
+          *reinterpret_cast<float4*>(&L.a[p][gk][c8]) = make_float4(w[p], w[p + 2], w[p + 4], w[p + 6]);
+          *reinterpret_cast<float4*>(&L.a[p][gk][c8 + 4]) = make_float4(w[p + 8], w[p + 10], w[p + 12], w[p + 14]);
+        } else {
+          bf16x8 hi, lo;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            if constexpr (PREC == HPX_PREC_BF16X3) {
+              __bf16 h, l;
+              split(w[p + 2 * q], h, l);
+              hi[q] = h;
+              lo[q] = l;
+            } else {
+              hi[q] = (__bf16)w[p + 2 * q];
+            }
+          }
+          const int slot = bf16_slot(gk, c8);
+          *reinterpret_cast<bf16x8*>(&L.hi[p][gk][slot]) = hi;
+          if constexpr (PREC == HPX_PREC_BF16X3) *reinterpret_cast<bf16x8*>(&L.lo[p][gk][slot]) = lo;
+        }
+      }
+      return;
+    }
     if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) L.a[sp][sr][sh + q] = v[q];
@@ -751,7 +869,7 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
     float b0[2][CT][8], b1[2][CT][8];
     load_a(0, pa);
     if (wave_active) load_b(0, b0);
-    store_a(lds[0], pa);
+    store_a(lds[0], pa, 0);
     __syncthreads();
     for (int c = 0; c < nchunk; c += 2) {
       if (c + 1 < nchunk) {
@@ -759,7 +877,7 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
         if (wave_active) load_b((c + 1) * kKTile, b1);
       }
       if (wave_active) mfma_chunk<PREC, CT>(lds[0], b0, acc, lane, rh * 64);
-      if (c + 1 < nchunk) store_a(lds[1], pa);
+      if (c + 1 < nchunk) store_a(lds[1], pa, (c + 1) * kKTile);
       __syncthreads();
       if (c + 1 >= nchunk) break;
       if (c + 2 < nchunk) {
@@ -767,7 +885,7 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
         if (wave_active) load_b((c + 2) * kKTile, b0);
       }
       if (wave_active) mfma_chunk<PREC, CT>(lds[1], b1, acc, lane, rh * 64);
-      if (c + 2 < nchunk) store_a(lds[0], pa);
+      if (c + 2 < nchunk) store_a(lds[0], pa, (c + 2) * kKTile);
       __syncthreads();
     }
   }
@@ -924,8 +1042,7 @@ int launch_legendre_table(ShtPlan& p, const double* d_ct, hipStream_t s) {
     vector_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.vstride, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
                                                   p.lmax, p.csphase, p.vnorm, p.vinverse);
   } else {
-    legendre_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
-                                                    p.lmax);
+    checkpoint_kernel<<<grid, kThreads, 0, s>>>(p.d_ck, p.d_seg_off, d_ct, p.npair, p.kpad, p.lmax);
   }
   HPX_LAUNCH_CHECK();
   if (p.d_pt != nullptr && p.mmax > 0) {
@@ -958,14 +1075,15 @@ int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, flo
   const int64_t ncb = (ncol + 64 * ct - 1) / (64 * ct);
   if ((int64_t)p.n_fwd_items * ncb > 0x7ffffff0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   const dim3 grid(xcd_grid((int64_t)p.n_fwd_items * ncb));
+  const OtfArgs otf{p.d_ck, p.d_ab, p.d_seg_off, p.d_ct};
 #define HPX_FWD(PR, CT)                                                                                       \
   do {                                                                                                        \
     if (p.vector)                                                                                             \
-      legendre_fwd_kernel<PR, CT, true><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, p.d_kmin,     \
+      legendre_fwd_kernel<PR, CT, true><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, otf, p.d_p_off, p.d_nl, p.d_kmin,     \
                                                                   p.d_fwd_items, p.n_fwd_items, (int)ncb, p.kpad, \
                                                                   ncol, p.vstride);                             \
     else                                                                                                      \
-      legendre_fwd_kernel<PR, CT><<<grid, kThreads, 0, s>>>(X, aT, p.d_p, p.d_p_off, p.d_nl, p.d_kmin,        \
+      legendre_fwd_kernel<PR, CT><<<grid, kThreads, 0, s>>>(X, aT, nullptr, otf, p.d_p_off, p.d_nl, p.d_kmin,   \
                                                             p.d_fwd_items, p.n_fwd_items, (int)ncb, p.kpad,   \
                                                             ncol, 0);                                         \
   } while (0)
@@ -994,14 +1112,15 @@ int launch_gemm_inv(const ShtPlan& p, int precision, int ct, int rh, const float
   const int nitems = nkb * p.mmax;
   if ((int64_t)nitems * ncb > 0x7ffffff0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   const dim3 grid(xcd_grid((int64_t)nitems * ncb));
+  const OtfArgs otf{p.d_ck, p.d_ab, p.d_seg_off, p.d_ct};
 #define HPX_INV(PR, CT, RH)                                                                                    \
   do {                                                                                                         \
     if (RH == 1 && p.vector)                                                                                   \
-      legendre_inv_kernel<PR, CT, 1, true><<<grid, kThreads, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, p.d_kmin, nkb, \
+      legendre_inv_kernel<PR, CT, 1, true><<<grid, kThreads, 0, s>>>(aT, X, p.d_pt, otf, p.d_p_off, p.d_nl, p.d_kmin, nkb, \
                                                                      nitems, (int)ncb, p.kpad, ncol, p.at_rows, \
                                                                      p.vstride);                                \
     else                                                                                                       \
-      legendre_inv_kernel<PR, CT, RH><<<grid, kThreads * RH, 0, s>>>(aT, X, p.d_pt, p.d_p_off, p.d_nl, p.d_kmin, nkb, \
+      legendre_inv_kernel<PR, CT, RH><<<grid, kThreads * RH, 0, s>>>(aT, X, nullptr, otf, p.d_p_off, p.d_nl, p.d_kmin, nkb, \
                                                                      nitems, (int)ncb, p.kpad, ncol, p.at_rows, 0); \
   } while (0)
   switch (precision * 8 + ct * 2 + rh - 1) {

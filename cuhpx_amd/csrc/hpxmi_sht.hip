@@ -90,6 +90,10 @@ void free_plan(ShtPlan* p) {
   hipFree(p->d_twiddle);
   hipFree(p->d_p);
   hipFree(p->d_pt);
+  hipFree(p->d_ck);
+  hipFree(p->d_ab);
+  hipFree(p->d_seg_off);
+  hipFree(p->d_ct);
   hipFree(p->d_p_off);
   hipFree(p->d_nl);
   hipFree(p->d_fwd_items);
@@ -257,6 +261,29 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
 
   std::vector<double> ct(p->kpad, 0.0);
   for (int k = 0; k < p->npair; ++k) ct[k] = cos_theta[k];
+  // Scalar plans: recurrence segments of 16 degrees per order, and the
+  // coefficients (a_l, b_l) of P_l = a_l x P_{l-1} - b_l P_{l-2}
+  // (sht_tools.py:250-251) per segment degree, FP64-computed, zero for the
+  // segment's first two degrees (stored values) and from lmax on.
+  std::vector<float2> ab;
+  std::vector<float> ctf;
+  if (!p->vector) {
+    p->seg_off.assign(mmax + 1, 0);
+    for (int m = 0; m < mmax; ++m) p->seg_off[m + 1] = p->seg_off[m] + nseg_of(lmax, m);
+    ab.assign((size_t)p->seg_off[mmax] * kSegL, make_float2(0.f, 0.f));
+    for (int m = 0; m < mmax; ++m)
+      for (int t = 0; t < nseg_of(lmax, m); ++t)
+        for (int j = 2; j < kSegL; ++j) {
+          const int l = m + kSegL * t + j;
+          if (l >= lmax) break;
+          const double dl = l, dm = m;
+          const double a = std::sqrt((2 * dl - 1) / (dl - dm) * (2 * dl + 1) / (dl + dm));
+          const double b = std::sqrt((dl + dm - 1) / (dl - dm) * (2 * dl + 1) / (2 * dl - 3) * (dl - dm - 1) / (dl + dm));
+          ab[((size_t)p->seg_off[m] + t) * kSegL + j] = make_float2((float)a, (float)b);
+        }
+    ctf.assign(p->kpad, 0.f);
+    for (int k = 0; k < p->npair; ++k) ctf[k] = (float)cos_theta[k];
+  }
 
   int64_t bytes = 0;
   int rc;
@@ -269,7 +296,10 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
       (rc = upload(&p->d_nl, p->nl, bytes)) != HPX_OK ||
       (rc = upload(&p->d_fwd_items, items, bytes)) != HPX_OK ||
       (rc = upload(&p->d_kmin, kmin, bytes)) != HPX_OK ||
-      (rc = upload(&p->d_plist, plist, bytes)) != HPX_OK) {
+      (rc = upload(&p->d_plist, plist, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_seg_off, p->seg_off, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_ab, ab, bytes)) != HPX_OK ||
+      (rc = upload(&p->d_ct, ctf, bytes)) != HPX_OK) {
     free_plan(p);
     return rc;
   }
@@ -284,21 +314,29 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
   const int64_t pfloats = p->p_off[mmax];
   p->at_rows = pfloats / p->kpad;
   p->vstride = p->vector ? pfloats : 0;
+  // vector plans: the two tables and their transposes; scalar plans: the
+  // segment checkpoints
   const int64_t tfloats = std::max<int64_t>((p->vector ? 2 : 1) * pfloats, 1);
+  const int64_t big_bytes = p->vector ? 2 * tfloats * (int64_t)sizeof(float)
+                                      : std::max<int64_t>((int64_t)p->seg_off[mmax] * p->kpad, 1) * (int64_t)sizeof(float2);
   {
-    // the tables are the plan's only large allocation: refuse it with the
-    // sizes named rather than fail inside hipMalloc
+    // the plan's only large allocation: refuse it with the sizes named
+    // rather than fail inside hipMalloc
     size_t free_b = 0, total_b = 0;
-    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess &&
-        (uint64_t)(2 * tfloats * sizeof(float)) > (uint64_t)free_b) {
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && (uint64_t)big_bytes > (uint64_t)free_b) {
       free_plan(p);
       HPX_FAIL(HPX_ERR_UNSUPPORTED, "plan tables need %.2f GB, %.2f GB free of %.2f GB on the device",
-               2.0 * tfloats * sizeof(float) / 1e9, free_b / 1e9, total_b / 1e9);
+               big_bytes / 1e9, free_b / 1e9, total_b / 1e9);
     }
   }
   double* d_ct = nullptr;
-  hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_p), tfloats * sizeof(float));
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->d_pt), tfloats * sizeof(float));
+  hipError_t e = hipSuccess;
+  if (p->vector) {
+    e = hipMalloc(reinterpret_cast<void**>(&p->d_p), tfloats * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&p->d_pt), tfloats * sizeof(float));
+  } else {
+    e = hipMalloc(reinterpret_cast<void**>(&p->d_ck), big_bytes);
+  }
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&d_ct), ct.size() * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(d_ct, ct.data(), ct.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -306,7 +344,7 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     free_plan(p);
     HPX_FAIL(HPX_ERR_HIP, "plan allocation failed: %s", hipGetErrorString(e));
   }
-  bytes += 2 * tfloats * (int64_t)sizeof(float);
+  bytes += big_bytes;
   rc = launch_legendre_table(*p, d_ct, stream);
   if (rc == HPX_OK) {
     e = hipStreamSynchronize(stream);

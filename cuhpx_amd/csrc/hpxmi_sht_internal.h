@@ -70,6 +70,16 @@ struct ShtPlan {
   float2* d_chirp = nullptr;
   float2* d_bhat = nullptr;
   float2* d_twiddle = nullptr;  // [max_M] exp(-2 pi i x / max_M)
+  // Scalar plans generate P_lm inside the GEMMs (no table): segment s of
+  // order m covers l = m + 16 s .. m + 16 s + 15; the FP64-computed pair
+  // (P_{l0}, P_{l0+1}) of every segment and ring pair starts an FP32
+  // three-term recurrence over the other 14 degrees.
+  float2* d_ck = nullptr;       // checkpoints [seg_off[m] + s][kpad]: (P_{l0}, P_{l0+1})
+  float2* d_ab = nullptr;       // recurrence coefficients [seg_off[m] + s][16]: (a_l, b_l)
+  int* d_seg_off = nullptr;     // [mmax + 1] first segment of each order
+  float* d_ct = nullptr;        // [kpad] cos(theta) of the north ring of each pair (0 beyond)
+  std::vector<int> seg_off;     // host copy
+  // Vector plans hold tables instead:
   float* d_p = nullptr;         // Legendre table [m][parity][row][kpad]
   float* d_pt = nullptr;        // the same transposed per (m, parity): [m][parity][k][row]
   int64_t* d_p_off = nullptr;   // [mmax + 1]
@@ -111,6 +121,9 @@ struct ShtPlan {
 
 // Rows of the P table per (m, parity) are padded to this many.
 constexpr int kRowPad = 16;
+// Degrees per recurrence segment of the scalar GEMMs (8 rows of each parity).
+constexpr int kSegL = 16;
+inline int nseg_of(int lmax, int m) { return m < lmax ? (lmax - m + kSegL - 1) / kSegL : 0; }
 // Ring-pair (K) tile of the Legendre GEMMs.
 constexpr int kKTile = 32;
 // Largest FFT length (complex points) of the ring FFT kernels.
