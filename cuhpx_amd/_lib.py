@@ -71,6 +71,7 @@ SIGNATURES = {
     "hpx_sht_max_batch": (_c_i64, [_vp]),
     "hpx_sht_forward": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_inverse": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_sht_forward_noalloc": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp]),
     "hpx_sht_forward_adjoint": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_inverse_adjoint": (_c_int, [_vp, _c_int, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_vsht_plan_create_healpix": (_c_int, [_c_int, _c_int, _c_int, _dp, _c_int, _c_int, _c_int, _vp,

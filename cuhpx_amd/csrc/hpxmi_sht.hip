@@ -17,7 +17,9 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <mutex>
 #include <string>
+#include <vector>
 
 namespace hpxmi {
 
@@ -521,6 +523,62 @@ extern "C" int hpx_sht_forward(const hpx_sht_plan* plan, int precision, const fl
   float* X = static_cast<float*>(workspace);
   if ((rc = launch_analysis(*pl, f, X, nbatch, 1, 1.0f, 0, stream)) != HPX_OK) return rc;
   return launch_legendre_fwd(*pl, precision, X, alm, nbatch, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Forward SHT without a caller workspace: the workspace is a fixed device
+// array of the library (its address looked up once per device), for callers
+// that cannot allocate device memory.
+// ---------------------------------------------------------------------------
+
+static constexpr int64_t kScratchBytes = int64_t(128) << 20;
+// Held by every device the library runs on until the process ends.  Host
+// code never uses this name as an address: scratch_ptr() gives the device's
+// address of its copy.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;  // per device ordinal
+
+// The current device's address of `scratch`, looked up on the first call on
+// that device.
+int scratch_ptr(void** out) {
+  int dev = 0;
+  HPX_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (dev >= (int)g_scratch_ptr.size()) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    HPX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (ptr == nullptr) HPX_FAIL(HPX_ERR_HIP, "no device address for the scratch array");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return HPX_OK;
+}
+}  // namespace
+
+extern "C" int hpx_sht_forward_noalloc(const hpx_sht_plan* plan, int precision, const float* f, float* alm,
+                                       int64_t nbatch, hipStream_t stream) {
+  const ShtPlan* pl = P(plan);
+  if (pl == nullptr) HPX_FAIL(HPX_ERR_ARG, "null plan");
+  if (nbatch < 0) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 0");
+  if (nbatch > max_maps(*pl))
+    HPX_FAIL(HPX_ERR_UNSUPPORTED, "%lld maps in one call exceed this plan's limit of %lld; split the batch",
+             (long long)nbatch, (long long)max_maps(*pl));
+  const int64_t need = workspace_floats(*pl, nbatch) * (int64_t)sizeof(float);
+  if (need > kScratchBytes)
+    HPX_FAIL(HPX_ERR_UNSUPPORTED,
+             "%lld maps need %lld bytes of workspace, more than the %lld-byte scratch of the "
+             "allocation-free entry point; split the batch or use hpx_sht_forward",
+             (long long)nbatch, (long long)need, (long long)kScratchBytes);
+  void* ws = nullptr;
+  const int rc = scratch_ptr(&ws);
+  if (rc != HPX_OK) return rc;
+  // every stage reads only workspace entries an earlier stage of the same
+  // call wrote, so nothing needs clearing between calls
+  return hpx_sht_forward(plan, precision, f, alm, nbatch, ws, kScratchBytes, stream);
 }
 
 extern "C" int hpx_sht_inverse(const hpx_sht_plan* plan, int precision, const float* alm, float* f,

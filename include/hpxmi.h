@@ -148,6 +148,19 @@ int hpx_sht_forward(const hpx_sht_plan* plan, int precision, const float* f,
                     float* alm, int64_t nbatch, void* workspace,
                     int64_t workspace_bytes, hipStream_t stream);
 
+/* hpx_sht_forward without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * workspace is a fixed 128 MiB device array of the library (one copy per
+ * device, held until the process ends); calls that need more
+ * (hpx_sht_workspace_bytes(plan, nbatch) > 134217728) return
+ * HPX_ERR_UNSUPPORTED before touching the device, e.g. nside 256 with
+ * lmax = mmax = 513 takes at most 16 maps per call.  Makes no allocation;
+ * calls that overlap on one device race on the array (serialise them, e.g.
+ * on one stream).  Results equal hpx_sht_forward's bit for bit. */
+int hpx_sht_forward_noalloc(const hpx_sht_plan* plan, int precision, const float* f,
+                            float* alm, int64_t nbatch, hipStream_t stream);
+
 /* Inverse SHT, alm[nbatch, lmax, mmax] -> f[nbatch, npix].
  * Replaces iSHTFunction.forward (hpx_sht.py:588-611): the einsum
  * '...lmn,mlk->...kmn' (hpx_sht.py:599) followed by cuhpx_fft
