@@ -18,7 +18,8 @@ PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 BUILD_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(PKG, "libhpxmi.so")
-SOURCES = ["hpxmi_remap.hip", "hpxmi_fft.hip", "hpxmi_legendre.hip", "hpxmi_sht.hip", "hpxmi_batch.hip"]
+SOURCES = ["hpxmi_remap.hip", "hpxmi_fft.hip", "hpxmi_legendre.hip", "hpxmi_sht.hip", "hpxmi_batch.hip",
+           "hpxmi_f64.hip"]
 ARCH = os.environ.get("HPXMI_ARCH", "gfx950")
 
 

@@ -34,16 +34,10 @@ namespace {
 
 using cd = std::complex<double>;
 
-bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
-
-int ilog2(int64_t n) {
-  int r = 0;
-  while ((int64_t(1) << r) < n) ++r;
-  return r;
-}
+}  // namespace
 
 // In-place iterative radix-2 FFT, exp(-2 pi i jk / n), n a power of two.
-void fft64(std::vector<cd>& a) {
+void host_fft(std::vector<std::complex<double>>& a) {
   const size_t n = a.size();
   for (size_t i = 1, j = 0; i < n; ++i) {
     size_t bit = n >> 1;
@@ -66,6 +60,16 @@ void fft64(std::vector<cd>& a) {
         a[i + k + half] = u - v;
       }
   }
+}
+
+namespace {
+
+bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+int ilog2(int64_t n) {
+  int r = 0;
+  while ((int64_t(1) << r) < n) ++r;
+  return r;
 }
 
 float2 to_f2(cd z) { return make_float2((float)z.real(), (float)z.imag()); }
@@ -92,6 +96,14 @@ void free_plan(ShtPlan* p) {
   hipFree(p->d_twiddle);
   hipFree(p->d_p);
   hipFree(p->d_pt);
+  hipFree(p->d_p64);
+  hipFree(p->d_pairs64);
+  hipFree(p->d_chirp64);
+  hipFree(p->d_bhat64);
+  hipFree(p->d_tw64);
+  hipFree(p->d_phase64);
+  hipFree(p->d_items64);
+  hipFree(p->d_w64);
   hipFree(p->d_ck);
   hipFree(p->d_ab);
   hipFree(p->d_seg_off);
@@ -175,7 +187,7 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
         b[j] = std::conj(w[j]);
         b[r.M - j] = std::conj(w[j]);
       }
-      fft64(b);
+      host_fft(b);
       for (int x = 0; x < r.M; ++x) bhat.push_back(to_f2(b[x] / (double)r.M));
     }
   }
@@ -263,6 +275,9 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
 
   std::vector<double> ct(p->kpad, 0.0);
   for (int k = 0; k < p->npair; ++k) ct[k] = cos_theta[k];
+  p->h_ct.assign(cos_theta, cos_theta + p->npair);
+  p->h_phi0.assign(phi0, phi0 + p->npair);
+  p->h_w.assign(weights, weights + p->npair);
   // Scalar plans: recurrence segments of 16 degrees per order, and the
   // coefficients (a_l, b_l) of P_l = a_l x P_{l-1} - b_l P_{l-2}
   // (sht_tools.py:250-251) per segment degree, FP64-computed, zero for the

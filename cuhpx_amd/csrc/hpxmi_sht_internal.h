@@ -4,6 +4,7 @@
 #include "hpxmi_common.h"
 
 #include <algorithm>
+#include <complex>
 #include <map>
 #include <mutex>
 #include <utility>
@@ -79,6 +80,20 @@ struct ShtPlan {
   int* d_seg_off = nullptr;     // [mmax + 1] first segment of each order
   float* d_ct = nullptr;        // [kpad] cos(theta) of the north ring of each pair (0 beyond)
   std::vector<int> seg_off;     // host copy
+  std::vector<double> h_ct, h_phi0, h_w;  // per ring pair: cos(theta), phi0, weight (host, FP64)
+  double* d_w64 = nullptr;                 // ring-pair weights (FP64 transforms)
+  // FP64 transforms (hpx_sht_plan_enable_f64): the Legendre table in FP64 in
+  // the layout of the FP32 table (p_off), Bluestein tables, twiddles and
+  // phase factors in FP64.
+  double* d_p64 = nullptr;
+  RingPair* d_pairs64 = nullptr;  // ring pairs with offsets into the FP64 Bluestein tables
+  double2* d_chirp64 = nullptr;
+  double2* d_bhat64 = nullptr;
+  double2* d_tw64 = nullptr;
+  double2* d_phase64 = nullptr;
+  int2* d_items64 = nullptr;      // forward FP64 GEMM items (m, parity * 65536 + row tile)
+  int n_items64 = 0;
+  int f64 = 0;
   // Vector plans hold tables instead:
   float* d_p = nullptr;         // Legendre table [m][parity][row][kpad]
   float* d_pt = nullptr;        // the same transposed per (m, parity): [m][parity][k][row]
@@ -154,6 +169,9 @@ inline int64_t max_maps(const ShtPlan& p) {
   const int64_t max_ncol = 0x7fffffff / (rows * (int64_t)sizeof(float)) / 32 * 32;
   return max_ncol / 2;
 }
+
+// In-place radix-2 FFT on the host (FP64), exp(-2 pi i jk / n), n a power of two.
+void host_fft(std::vector<std::complex<double>>& a);
 
 // Launchers (return HPX_OK or an error code after setting the error text).
 // layout 0: folded spectra workspace X[m][parity][kpad][ncol];

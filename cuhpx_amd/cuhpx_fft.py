@@ -12,7 +12,9 @@ irfft(ftm[t] e^{i m phi0_t}, n = N_t, norm = "forward")
 kernels multiply the phase into it in place, hpx_fft.cpp:135, 225).
 
 Both run the ring-pair FFT kernels of libhpxmi.so (Stockham in LDS, Bluestein
-for non-power-of-two rings) on an FFT-only plan (no Legendre table).
+for non-power-of-two rings) on an FFT-only plan (no Legendre table).  As in the
+reference (hpx_fft.cpp:45), float64 maps give complex128 spectra and complex128
+spectra float64 maps, computed in FP64.
 """
 
 import torch
@@ -26,8 +28,8 @@ def _plan(L, nside, device):
 
 
 def healpix_rfft_batch(f: torch.Tensor, L: int, nside: int) -> torch.Tensor:
-    if f.dtype != torch.float32:
-        raise TypeError(f"float32 maps only (got {f.dtype})")
+    if f.dtype not in (torch.float32, torch.float64):
+        raise TypeError(f"float32 or float64 maps only (got {f.dtype})")
     npix = 12 * nside * nside
     if f.shape[-1] != npix:
         raise RuntimeError(f"last dimension must be {npix}")
@@ -35,17 +37,21 @@ def healpix_rfft_batch(f: torch.Tensor, L: int, nside: int) -> torch.Tensor:
         raise RuntimeError("needs a GPU tensor")
     plan = _plan(L, nside, f.device)
     rows = f.reshape(-1, npix).contiguous()
-    out = torch.empty((rows.shape[0], 4 * nside - 1, L), dtype=torch.complex64, device=f.device)
+    f64 = f.dtype == torch.float64
+    if f64:
+        plan.enable_f64()
+    out = torch.empty((rows.shape[0], 4 * nside - 1, L), dtype=torch.complex128 if f64 else torch.complex64,
+                      device=f.device)
+    fn = _lib.lib().hpx_ring_rfft_f64 if f64 else _lib.lib().hpx_ring_rfft
     with torch.cuda.device(f.device):
-        rc = _lib.lib().hpx_ring_rfft(plan.handle, _lib.ptr(rows), _lib.ptr(out), rows.shape[0],
-                                      _lib.stream_ptr(f.device))
+        rc = fn(plan.handle, _lib.ptr(rows), _lib.ptr(out), rows.shape[0], _lib.stream_ptr(f.device))
     _lib.check(rc, "healpix_rfft")
     return out.reshape(*f.shape[:-1], 4 * nside - 1, L)
 
 
 def healpix_irfft_batch(ftm: torch.Tensor, L: int, nside: int) -> torch.Tensor:
-    if ftm.dtype != torch.complex64:
-        raise TypeError(f"complex64 spectra only (got {ftm.dtype})")
+    if ftm.dtype not in (torch.complex64, torch.complex128):
+        raise TypeError(f"complex64 or complex128 spectra only (got {ftm.dtype})")
     nring = 4 * nside - 1
     if tuple(ftm.shape[-2:]) != (nring, L):
         raise RuntimeError(f"last dimensions must be ({nring}, {L})")
@@ -54,10 +60,13 @@ def healpix_irfft_batch(ftm: torch.Tensor, L: int, nside: int) -> torch.Tensor:
     plan = _plan(L, nside, ftm.device)
     rows = ftm.reshape(-1, nring, L).contiguous()
     npix = 12 * nside * nside
-    out = torch.empty((rows.shape[0], npix), dtype=torch.float32, device=ftm.device)
+    f64 = ftm.dtype == torch.complex128
+    if f64:
+        plan.enable_f64()
+    out = torch.empty((rows.shape[0], npix), dtype=torch.float64 if f64 else torch.float32, device=ftm.device)
+    fn = _lib.lib().hpx_ring_irfft_f64 if f64 else _lib.lib().hpx_ring_irfft
     with torch.cuda.device(ftm.device):
-        rc = _lib.lib().hpx_ring_irfft(plan.handle, _lib.ptr(rows), _lib.ptr(out), rows.shape[0],
-                                       _lib.stream_ptr(ftm.device))
+        rc = fn(plan.handle, _lib.ptr(rows), _lib.ptr(out), rows.shape[0], _lib.stream_ptr(ftm.device))
     _lib.check(rc, "healpix_irfft")
     return out.reshape(*ftm.shape[:-2], npix)
 

@@ -122,8 +122,8 @@ class LatLonSHT(_LatLonBase):
     (torch_harmonics.RealSHT semantics with norm='ortho')."""
 
     def forward(self, x):
-        if torch.is_complex(x) or x.dtype != torch.float32:
-            raise TypeError("LatLonSHT takes float32 maps")
+        if torch.is_complex(x) or x.dtype not in (torch.float32, torch.float64):
+            raise TypeError("LatLonSHT takes float32 or float64 maps")
         plan = self.plan_for(x.device)
         lead = x.shape[:-2]
         flat = x.reshape(*lead, self.nlat * self.nlon)
@@ -135,8 +135,8 @@ class InverseLatLonSHT(_LatLonBase):
     (torch_harmonics.InverseRealSHT semantics with norm='ortho')."""
 
     def forward(self, a):
-        if a.dtype != torch.complex64:
-            raise TypeError("InverseLatLonSHT takes complex64 coefficients")
+        if a.dtype not in (torch.complex64, torch.complex128):
+            raise TypeError("InverseLatLonSHT takes complex64 or complex128 coefficients")
         plan = self.plan_for(a.device)
         out = iSHTFunction.apply(a, plan, self.precision, self.adjoint)
         return out.reshape(*a.shape[:-2], self.nlat, self.nlon)

@@ -19,7 +19,9 @@ Differences, all deliberate:
   * extra keyword ``precision`` selects the Legendre arithmetic: "fp32"
     (default, FP32 MFMA), "bf16x3" (3-term split BF16 MFMA, FP32 accumulate)
     or "bf16" (single-term BF16);
-  * float64 input is not supported (TypeError).
+  * float64 input computes in FP64 end to end (FP64 ring FFTs, FP64 table and
+    GEMMs; ``precision`` applies to float32 input only), returning complex128,
+    as the reference does (hpx_fft.cpp:45; hpx_sht.py:557, 599).
 
 All device work is in libhpxmi.so (ring FFTs, Legendre GEMMs, table
 generation); this module only allocates outputs and the workspace.
@@ -66,6 +68,7 @@ class ShtPlan:
         self.mmax = mmax
         self.vector = vector
         self.max_batch = int(_lib.lib().hpx_sht_max_batch(handle))
+        self._f64 = False
 
     @property
     def handle(self):
@@ -154,11 +157,24 @@ class ShtPlan:
     def table_bytes(self):
         return int(_lib.lib().hpx_sht_plan_table_bytes(self._handle))
 
-    def workspace(self, nbatch):
-        """Workspace for nbatch maps (scalar plan) or nbatch vector fields."""
-        fn = _lib.lib().hpx_vsht_workspace_bytes if self.vector else _lib.lib().hpx_sht_workspace_bytes
+    def workspace(self, nbatch, f64=False):
+        """Workspace for nbatch maps (scalar plan) or nbatch vector fields;
+        ``f64``: for the FP64 calls."""
+        if f64:
+            fn = _lib.lib().hpx_sht_workspace_bytes_f64
+        else:
+            fn = _lib.lib().hpx_vsht_workspace_bytes if self.vector else _lib.lib().hpx_sht_workspace_bytes
         nbytes = int(fn(self._handle, nbatch))
         return torch.empty(max(nbytes, 1), dtype=torch.uint8, device=self.device), nbytes
+
+    def enable_f64(self):
+        """Build the plan's FP64 tables (once; allocates and synchronises)."""
+        with _plan_lock:
+            if not self._f64:
+                with torch.cuda.device(self.device):
+                    rc = _lib.lib().hpx_sht_plan_enable_f64(self._handle, _lib.stream_ptr(self.device))
+                _lib.check(rc, "FP64 plan tables")
+                self._f64 = True
 
     def autotune(self, precision, nbatch):
         """Measure and keep the fastest FFT / GEMM launch shapes for this
@@ -180,36 +196,48 @@ class ShtPlan:
                 "gemm_inv_col_tiles": v[7], "gemm_inv_row_blocks": v[8],
                 "fft_analysis_threads_per_field": v[9:12], "fft_synthesis_threads_per_field": v[12:15]}
 
-    def _run(self, fn, *args, nbatch):
-        ws, nbytes = self.workspace(nbatch)
+    def _run(self, fn, *args, nbatch, f64=False):
+        ws, nbytes = self.workspace(nbatch, f64)
         with torch.cuda.device(self.device):
             rc = fn(self._handle, *args, nbatch, _lib.ptr(ws), nbytes, _lib.stream_ptr(self.device))
         return rc
 
-    def _call(self, fn, head, inp, out, what):
+    def _call(self, fn, head, inp, out, what, f64=False):
         """fn(handle, *head, in, out, nbatch, workspace, bytes, stream) over the
         rows of the contiguous inp and out, at most max_batch rows per call
-        (one call, with nbatch 0, for an empty batch)."""
+        (half as many in FP64; one call, with nbatch 0, for an empty batch)."""
         n = inp.shape[0]
-        step = max(1, self.max_batch)
+        step = max(1, self.max_batch // 2 if f64 else self.max_batch)
         ib = inp.stride(0) * inp.element_size()
         ob = out.stride(0) * out.element_size()
         for s in range(0, max(n, 1), step):
             c = min(step, n - s)
             rc = self._run(fn, *head, ctypes.c_void_p(inp.data_ptr() + s * ib),
-                           ctypes.c_void_p(out.data_ptr() + s * ob), nbatch=c)
+                           ctypes.c_void_p(out.data_ptr() + s * ob), nbatch=c, f64=f64)
             _lib.check(rc, what)
 
     def forward(self, f, precision):
-        """f [n, npix] float32 contiguous -> alm [n, lmax, mmax] complex64."""
+        """f [n, npix] float32 (or float64) contiguous -> alm [n, lmax, mmax]
+        complex64 (complex128, computed in FP64)."""
         n = f.shape[0]
+        if f.dtype == torch.float64:
+            self.enable_f64()
+            out = torch.empty((n, self.lmax, self.mmax), dtype=torch.complex128, device=f.device)
+            self._call(_lib.lib().hpx_sht_forward_f64, (), f, out, "SHT forward (FP64)", f64=True)
+            return out
         out = torch.empty((n, self.lmax, self.mmax), dtype=torch.complex64, device=f.device)
         self._call(_lib.lib().hpx_sht_forward, (_lib.PRECISIONS[precision],), f, out, "SHT forward")
         return out
 
     def inverse(self, alm, precision):
-        """alm [n, lmax, mmax] complex64 contiguous -> f [n, npix] float32."""
+        """alm [n, lmax, mmax] complex64 (or complex128) contiguous -> f [n, npix]
+        float32 (float64, computed in FP64)."""
         n = alm.shape[0]
+        if alm.dtype == torch.complex128:
+            self.enable_f64()
+            out = torch.empty((n, self.npix), dtype=torch.float64, device=alm.device)
+            self._call(_lib.lib().hpx_sht_inverse_f64, (), alm, out, "SHT inverse (FP64)", f64=True)
+            return out
         out = torch.empty((n, self.npix), dtype=torch.float32, device=alm.device)
         self._call(_lib.lib().hpx_sht_inverse, (_lib.PRECISIONS[precision],), alm, out, "SHT inverse")
         return out
@@ -230,6 +258,12 @@ class ShtPlan:
 
     def forward_adjoint(self, galm, precision, mode):
         n = galm.shape[0]
+        if galm.dtype == torch.complex128:
+            self.enable_f64()
+            out = torch.empty((n, self.npix), dtype=torch.float64, device=galm.device)
+            self._call(_lib.lib().hpx_sht_forward_adjoint_f64, (_lib.ADJOINT_MODES[mode],), galm, out,
+                       "SHT forward adjoint (FP64)", f64=True)
+            return out
         out = torch.empty((n, self.npix), dtype=torch.float32, device=galm.device)
         self._call(_lib.lib().hpx_sht_forward_adjoint, (_lib.PRECISIONS[precision], _lib.ADJOINT_MODES[mode]),
                    galm, out, "SHT forward adjoint")
@@ -237,6 +271,12 @@ class ShtPlan:
 
     def inverse_adjoint(self, gf, precision, mode):
         n = gf.shape[0]
+        if gf.dtype == torch.float64:
+            self.enable_f64()
+            out = torch.empty((n, self.lmax, self.mmax), dtype=torch.complex128, device=gf.device)
+            self._call(_lib.lib().hpx_sht_inverse_adjoint_f64, (_lib.ADJOINT_MODES[mode],), gf, out,
+                       "SHT inverse adjoint (FP64)", f64=True)
+            return out
         out = torch.empty((n, self.lmax, self.mmax), dtype=torch.complex64, device=gf.device)
         self._call(_lib.lib().hpx_sht_inverse_adjoint, (_lib.PRECISIONS[precision], _lib.ADJOINT_MODES[mode]),
                    gf, out, "SHT inverse adjoint")
@@ -259,6 +299,7 @@ class SHTFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, plan, precision, adjoint):
         ctx.plan, ctx.precision, ctx.adjoint = plan, precision, adjoint
+        ctx.cdtype = torch.complex128 if x.dtype == torch.float64 else torch.complex64
         rows, lead = _as_rows(x, plan.npix)
         out = plan.forward(rows, precision)
         return out.reshape(*lead, plan.lmax, plan.mmax)
@@ -266,7 +307,7 @@ class SHTFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad):
         plan = ctx.plan
-        rows, lead = _as_alm_rows(grad.to(torch.complex64), plan.lmax, plan.mmax)
+        rows, lead = _as_alm_rows(grad.to(ctx.cdtype), plan.lmax, plan.mmax)
         gx = plan.forward_adjoint(rows, ctx.precision, ctx.adjoint)
         return gx.reshape(*lead, plan.npix), None, None, None
 
@@ -277,6 +318,7 @@ class iSHTFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, a, plan, precision, adjoint):
         ctx.plan, ctx.precision, ctx.adjoint = plan, precision, adjoint
+        ctx.rdtype = torch.float64 if a.dtype == torch.complex128 else torch.float32
         rows, lead = _as_alm_rows(a, plan.lmax, plan.mmax)
         out = plan.inverse(rows, precision)
         return out.reshape(*lead, plan.npix)
@@ -284,7 +326,7 @@ class iSHTFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad):
         plan = ctx.plan
-        rows, lead = _as_rows(grad.to(torch.float32), plan.npix)
+        rows, lead = _as_rows(grad.to(ctx.rdtype), plan.npix)
         ga = plan.inverse_adjoint(rows, ctx.precision, ctx.adjoint)
         return ga.reshape(*lead, plan.lmax, plan.mmax), None, None, None
 
@@ -335,8 +377,8 @@ class SHTCUDA(_HealpixSHTBase):
     def forward(self, x):
         if torch.is_complex(x):
             raise ValueError("Input tensor must be real.")
-        if x.dtype != torch.float32:
-            raise TypeError(f"SHTCUDA computes in float32; got {x.dtype}")
+        if x.dtype not in (torch.float32, torch.float64):
+            raise TypeError(f"SHTCUDA takes float32 or float64 maps; got {x.dtype}")
         plan = self.plan_for(x.device)
         if x.shape[-1] != plan.npix:
             raise RuntimeError(f"last dimension must be {plan.npix}, got {x.shape[-1]}")
@@ -349,8 +391,8 @@ class iSHTCUDA(_HealpixSHTBase):
     def forward(self, x):
         if not torch.is_complex(x):
             raise ValueError("Input tensor must be complex.")
-        if x.dtype != torch.complex64:
-            raise TypeError(f"iSHTCUDA computes in complex64; got {x.dtype}")
+        if x.dtype not in (torch.complex64, torch.complex128):
+            raise TypeError(f"iSHTCUDA takes complex64 or complex128 coefficients; got {x.dtype}")
         plan = self.plan_for(x.device)
         if tuple(x.shape[-2:]) != (plan.lmax, plan.mmax):
             raise RuntimeError(f"last dimensions must be ({plan.lmax}, {plan.mmax}), got {tuple(x.shape[-2:])}")

@@ -185,6 +185,50 @@ int hpx_sht_inverse_adjoint(const hpx_sht_plan* plan, int precision, int mode,
                             hipStream_t stream);
 
 /* ------------------------------------------------------------------ */
+/* FP64 transforms (float64 maps, complex128 coefficients)             */
+/* ------------------------------------------------------------------ */
+
+/* The reference computes in FP64 when given float64 maps: its FFT dispatch
+ * returns complex128 spectra for float64 input (src/harmonic_transform/
+ * hpx_fft.cpp:45) and the einsums cast the FP64 table to the input's dtype
+ * (cuhpx/hpx_sht.py:557, 599).  These calls are that path: FP64 ring FFTs
+ * (Bluestein tables in FP64) and FP64 Legendre GEMMs on an FP64 table.
+ *
+ * hpx_sht_plan_enable_f64 builds a scalar plan's FP64 state (the FP64
+ * Legendre table in HBM: 8 bytes per (m, l, ring pair) of the folded
+ * triangle, 0.6 GB at nside 256, lmax = mmax = 513; Bluestein tables,
+ * twiddles, phases); it allocates, synchronises the stream, and is a no-op
+ * on a plan that has it.  The _f64 calls below return HPX_ERR_ARG on a plan
+ * without it.  Rings longer than 8192 points (nside > 1024) are refused. */
+int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream);
+
+/* Workspace bytes of an FP64 call on nbatch maps. */
+int64_t hpx_sht_workspace_bytes_f64(const hpx_sht_plan* plan, int64_t nbatch);
+
+/* hpx_sht_forward / hpx_sht_inverse / the adjoints in FP64: f float64
+ * [nbatch, npix], alm complex128 [nbatch, lmax, mmax] (interleaved re, im). */
+int hpx_sht_forward_f64(const hpx_sht_plan* plan, const double* f, double* alm,
+                        int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                        hipStream_t stream);
+int hpx_sht_inverse_f64(const hpx_sht_plan* plan, const double* alm, double* f,
+                        int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                        hipStream_t stream);
+int hpx_sht_forward_adjoint_f64(const hpx_sht_plan* plan, int mode, const double* galm,
+                                double* gf, int64_t nbatch, void* workspace,
+                                int64_t workspace_bytes, hipStream_t stream);
+int hpx_sht_inverse_adjoint_f64(const hpx_sht_plan* plan, int mode, const double* gf,
+                                double* galm, int64_t nbatch, void* workspace,
+                                int64_t workspace_bytes, hipStream_t stream);
+
+/* hpx_ring_rfft / hpx_ring_irfft in FP64 (complex128 ftm[nbatch, nring, mmax]);
+ * they replace cuhpx_fft.healpix_rfft_* / healpix_irfft_* on float64 /
+ * complex128 tensors (hpx_fft.cpp:45, 122). */
+int hpx_ring_rfft_f64(const hpx_sht_plan* plan, const double* f, double* ftm, int64_t nbatch,
+                      hipStream_t stream);
+int hpx_ring_irfft_f64(const hpx_sht_plan* plan, const double* ftm, double* f, int64_t nbatch,
+                       hipStream_t stream);
+
+/* ------------------------------------------------------------------ */
 /* Vector (spheroidal / toroidal) SHT                                  */
 /* ------------------------------------------------------------------ */
 

@@ -220,7 +220,7 @@ def test_errors():
     with pytest.raises(ValueError):
         sht(torch.zeros(192, dtype=torch.complex64, device="cuda"))
     with pytest.raises(TypeError):
-        sht(torch.zeros(192, dtype=torch.float64, device="cuda"))
+        sht(torch.zeros(192, dtype=torch.float16, device="cuda"))
     with pytest.raises(RuntimeError):
         sht(torch.zeros(191, device="cuda"))
 
