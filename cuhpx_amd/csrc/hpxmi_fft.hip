@@ -29,8 +29,8 @@
 // Bluestein's pointwise steps run inside the first and last Stockham passes,
 // and the first pass of an analysis reads the map (the last pass of a
 // synthesis writes it) directly.
-// Rings that need an 8192-point FFT (nside > 512) use a work-group-wide
-// variant of the same Stockham passes.
+// Rings that need an 8192- or 16384-point FFT (nside > 512) use a
+// work-group-wide variant of the same Stockham passes (up to 128 KB of LDS).
 
 #include "hpxmi_sht_internal.h"
 #include "../../include/hpxmi.h"
@@ -644,8 +644,8 @@ __global__ __launch_bounds__(WAVES * TPF) void ring_synthesis_wave_kernel(const 
 }
 
 // ---------------------------------------------------------------------------
-// Work-group-wide FFT for the 8192-point class: one ring pair x one field per
-// work-group, Stockham passes across 512 threads with barriers.
+// Work-group-wide FFT for the 8192- and 16384-point class: one ring pair x
+// one field per work-group, Stockham passes across 512 threads with barriers.
 // ---------------------------------------------------------------------------
 
 constexpr int kBlockThreads = 512;
@@ -717,7 +717,7 @@ __device__ void block_ring_dft(float2* buf, const RingPair& pr, const float2* __
 
 __global__ __launch_bounds__(kBlockThreads) void ring_analysis_block_kernel(
     const float* __restrict__ f, float* __restrict__ X, StageArgs a) {
-  __shared__ float2 buf[kFftLds];
+  extern __shared__ float2 buf[];  // the class's longest FFT
   const int64_t item = blockIdx.x;
   const int k = a.plist[item / a.nbatch];
   const int64_t b = item % a.nbatch;
@@ -764,7 +764,7 @@ __global__ __launch_bounds__(kBlockThreads) void ring_analysis_block_kernel(
 
 __global__ __launch_bounds__(kBlockThreads) void ring_synthesis_block_kernel(
     const float* __restrict__ X, float* __restrict__ f, StageArgs a) {
-  __shared__ float2 buf[kFftLds];
+  extern __shared__ float2 buf[];  // the class's longest FFT
   const int64_t item = blockIdx.x;
   const int k = a.plist[item / a.nbatch];
   const int64_t b = item % a.nbatch;
@@ -859,6 +859,10 @@ void launch_wave(bool analysis, unsigned grid, size_t lds_bytes, const float* in
 
 }  // namespace
 
+size_t fft_lds_bytes(const ShtPlan& p, int c, int waves) {
+  return ((size_t)waves * wave_stride(1 << (10 + c)) + p.mmax) * sizeof(float2);
+}
+
 int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, int tpf, const float* in,
                      float* out, int64_t nbatch, int use_weight, float scale, int cmode,
                      hipStream_t s, int layout) {
@@ -869,10 +873,20 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, int tpf,
   if (c == kFftClasses - 1) {
     const int64_t items = (int64_t)n * nbatch;
     if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+    // every pair of the class fits the plan's longest FFT (at most kFftLds points)
+    const size_t lds = (size_t)p.max_M * sizeof(float2);
+    static const bool attr_set = [] {
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(ring_analysis_block_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      (void)hipFuncSetAttribute(reinterpret_cast<const void*>(ring_synthesis_block_kernel),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      return true;
+    }();
+    (void)attr_set;
     if (analysis)
-      ring_analysis_block_kernel<<<(unsigned)items, kBlockThreads, 0, s>>>(in, out, a);
+      ring_analysis_block_kernel<<<(unsigned)items, kBlockThreads, lds, s>>>(in, out, a);
     else
-      ring_synthesis_block_kernel<<<(unsigned)items, kBlockThreads, 0, s>>>(in, out, a);
+      ring_synthesis_block_kernel<<<(unsigned)items, kBlockThreads, lds, s>>>(in, out, a);
     HPX_LAUNCH_CHECK();
     return HPX_OK;
   }
@@ -884,7 +898,7 @@ int launch_fft_class(const ShtPlan& p, bool analysis, int c, int waves, int tpf,
   const int64_t nfg = (nbatch + waves - 1) / waves;
   const int64_t items = (int64_t)n * nfg;
   if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  const size_t lds_bytes = ((size_t)waves * wave_stride(1 << (10 + c)) + p.mmax) * sizeof(float2);
+  const size_t lds_bytes = fft_lds_bytes(p, c, waves);
   if (lds_bytes > 160 * 1024) HPX_FAIL(HPX_ERR_UNSUPPORTED, "mmax too large for the ring FFT kernels");
   const unsigned grid = (unsigned)items;
 #define HPX_W(W, C, T) \
@@ -914,8 +928,12 @@ int launch_stage(const ShtPlan& p, const float* in, float* out, int64_t nbatch, 
   }
   const FftTune t = p.fft_tune(col_pad(nbatch));
   for (int c = 0; c < kFftClasses; ++c) {
-    const int waves = c < 3 ? t.waves[ANALYSIS ? 0 : 1][c] : 1;
+    int waves = c < 3 ? t.waves[ANALYSIS ? 0 : 1][c] : 1;
     const int tpf = c < 3 ? t.tpf[ANALYSIS ? 0 : 1][c] : 64;
+    // a large mmax (its phase factors share the work-group's LDS) leaves
+    // room for fewer fields per work-group: take the largest that fits
+    for (int j = 2; c < 3 && j > 0 && waves == kFftWaveChoices[c][j]; --j)
+      if (fft_lds_bytes(p, c, waves) > 160 * 1024) waves = kFftWaveChoices[c][j - 1];
     const int rc = launch_fft_class(p, ANALYSIS, c, waves, tpf, in, out, nbatch, use_weight, scale, cmode, s,
                                     layout);
     if (rc != HPX_OK) return rc;

@@ -141,10 +141,11 @@ constexpr int kSegL = 16;
 inline int nseg_of(int lmax, int m) { return m < lmax ? (lmax - m + kSegL - 1) / kSegL : 0; }
 // Ring-pair (K) tile of the Legendre GEMMs.
 constexpr int kKTile = 32;
-// Largest FFT length (complex points) of the ring FFT kernels.
-constexpr int kFftLds = 8192;
+// Largest FFT length (complex points) of the ring FFT kernels: 128 KB of
+// LDS, the 16384-point Bluestein rings of nside 2048.
+constexpr int kFftLds = 16384;
 // FFT size classes: log2 M <= 10, 11, 12 (1, 2 or 4 wavefronts per
-// transform) and 13 (one work-group per transform).
+// transform) and 13, 14 (one work-group per transform).
 constexpr int kFftClasses = 4;
 int fft_class_of(int log2M);
 
@@ -180,6 +181,9 @@ int launch_analysis(const ShtPlan& p, const float* f, float* X, int64_t nbatch,
                     int use_weight, float scale, int cmode, hipStream_t s, int layout = 0);
 int launch_synthesis(const ShtPlan& p, const float* X, float* f, int64_t nbatch,
                      int use_weight, float scale, int cmode, hipStream_t s, int layout = 0);
+// LDS bytes of a wave-level FFT work-group of size class c (< 3) with
+// `waves` fields (their sequences and the pair's mmax phase factors).
+size_t fft_lds_bytes(const ShtPlan& p, int c, int waves);
 // One size class of the ring FFT stage with an explicit work-group shape.
 int launch_fft_class(const ShtPlan& p, bool analysis, int cls, int waves, int tpf, const float* in,
                      float* out, int64_t nbatch, int use_weight, float scale, int cmode,

@@ -279,9 +279,48 @@ def test_nside1024_default_band_limit():
 
 
 def test_unsupported_nside_is_a_clean_error():
-    """nside 2048 needs 16384-point Bluestein FFTs: plan creation refuses it."""
+    """nside 4096 needs 32768-point FFTs, more than LDS holds: plan creation
+    refuses it."""
     with pytest.raises(NotImplementedError):
-        ShtPlan.healpix(2048, 9, 9)
+        ShtPlan.healpix(4096, 9, 9)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_nside2048_all_fft_classes(precision):
+    """nside 2048 (50 million pixels; rings of 4..8192 pixels, among them the
+    16384-point Bluestein rings of the work-group-wide FFT kernel), lmax =
+    mmax = 129, 2 fields, against the streamed FP64 oracle."""
+    nside, L = 2048, 129
+    print(f"[nside2048 {precision}] oracle", flush=True)
+    o = S.SHTOracle.healpix(nside, L, L, streamed=True)
+    alm = band_limited_alm(L, L, 2, 9)
+    f64 = o.inverse(alm)
+    f32 = f64.astype(np.float32)
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=L, precision=precision)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=L, precision=precision)
+    a = sht(torch.from_numpy(f32).cuda()).cpu().numpy()
+    back = isht(torch.from_numpy(alm.astype(np.complex64)).cuda()).cpu().numpy()
+    a_ref, g_ref = o.forward_and_inverse(f32.astype(np.float64), alm.astype(np.complex64).astype(np.complex128))
+    err_f, err_i = rel_l2(a, a_ref), rel_l2(back, g_ref)
+    print(f"[nside2048 {precision}] forward rel-L2 {err_f:.3e}  inverse rel-L2 {err_i:.3e}", flush=True)
+    assert err_f < TOL[precision] and err_i < TOL[precision]
+
+
+def test_nside2048_default_band_limit_plan():
+    """nside 2048 at the reference's default band limit (lmax = 8191, mmax =
+    4097): the plan holds 52 GB of recurrence checkpoints (a stored FP32 table
+    would take 412 GB, more than the device's memory)."""
+    import cuhpx_amd.hpx_sht as H
+    plan = ShtPlan.healpix(2048, 8191, 4097)
+    try:
+        print(f"[nside2048 L8191 M4097] plan {plan.table_bytes / 1e9:.1f} GB", flush=True)
+        assert 45e9 < plan.table_bytes < 60e9
+    finally:
+        key = [k for k, v in H._plans.items() if v is plan]
+        del plan
+        torch.cuda.synchronize()
+        assert not any(k in H._plans for k in key)
 
 
 def test_empty_batches_pass_through():
