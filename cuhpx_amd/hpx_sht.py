@@ -146,7 +146,8 @@ class ShtPlan:
 
     def __del__(self):
         h = getattr(self, "_handle", None)
-        if h is not None and h.value and not _lib.exiting():
+        # once the interpreter exits (or this module is torn down) no native call
+        if h is not None and h.value and getattr(_lib, "_exiting", True) is False:
             try:
                 _lib.lib().hpx_sht_plan_destroy(h)
             except Exception:  # interpreter shutdown
