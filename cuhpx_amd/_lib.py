@@ -80,6 +80,9 @@ SIGNATURES = {
     "hpx_sht_inverse_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_forward_adjoint_f64": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_inverse_adjoint_f64": (_c_int, [_vp, _c_int, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_vsht_workspace_bytes_f64": (_c_i64, [_vp, _c_i64]),
+    "hpx_vsht_forward_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
+    "hpx_vsht_inverse_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_ring_rfft_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _vp]),
     "hpx_ring_irfft_f64": (_c_int, [_vp, _vp, _vp, _c_i64, _vp]),
     "hpx_vsht_plan_create_healpix": (_c_int, [_c_int, _c_int, _c_int, _dp, _c_int, _c_int, _c_int, _vp,

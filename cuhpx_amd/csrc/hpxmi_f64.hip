@@ -337,15 +337,27 @@ __global__ __launch_bounds__(kT64) void ring_synthesis64_kernel(const double* __
 // threads with 2 x 4 outputs each, reduction in chunks of 16 through LDS.
 // ---------------------------------------------------------------------------
 
+// Column signs of the vector transform: columns come in fours per field,
+// (component, re / im); the D0 half of the reduction takes column c with sign
+// (+, +, -, -), the D1 half column c ^ 3 with sign (+, -, +, -) forward and
+// (-, +, -, +) inverse (hpx_sht.py:351-372 and 413-432 written out per real
+// column, as in the FP32 vector GEMMs).
+__device__ __forceinline__ double vsign0(int64_t c) { return (c & 2) ? -1.0 : 1.0; }
+__device__ __forceinline__ double vsign1(int64_t c, bool forward) { return ((c & 1) != 0) == forward ? -1.0 : 1.0; }
+
 // Forward: aT[row(m, p, i)][col] = sum_{k < npair} P[m][p][i][k] X[m][p][k][col]
 // for rows i < nl[m][p]; items (m, p << 16 | row tile), blockIdx.y = column tile.
+// VEC (vector transform): the sum over k runs twice, the D0 table (at P)
+// against the opposite parity's spectra, then the D1 table (at P + vstride)
+// against the same parity's, with the column map and signs above.
+template <bool VEC>
 __global__ __launch_bounds__(kGemmThreads) void gemm_fwd64_kernel(const double* __restrict__ X,
                                                                   double* __restrict__ aT,
                                                                   const double* __restrict__ P,
                                                                   const int64_t* __restrict__ p_off,
                                                                   const int* __restrict__ nl,
                                                                   const int2* __restrict__ items, int kpad,
-                                                                  int npair, int64_t ncol) {
+                                                                  int npair, int64_t ncol, int64_t vstride) {
   __shared__ double As[32][17];
   __shared__ double Bs[16][65];
   const int2 it = items[blockIdx.x];
@@ -353,10 +365,12 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_fwd64_kernel(const double* 
   const int64_t c0 = (int64_t)blockIdx.y * 64;
   const int nlp = nl[2 * m + p];
   const int pad0 = (nl[2 * m] + kRowPad - 1) / kRowPad * kRowPad;
-  const double* Pm = P + p_off[m] + (p ? (int64_t)pad0 * kpad : 0);
-  const double* Xm = X + ((int64_t)m * 2 + p) * kpad * ncol;
   const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
   double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  for (int half = 0; half < (VEC ? 2 : 1); ++half) {
+  const double* Pm = P + half * vstride + p_off[m] + (p ? (int64_t)pad0 * kpad : 0);
+  const int pb = VEC && half == 0 ? 1 - p : p;  // spectra parity block
+  const double* Xm = X + ((int64_t)m * 2 + pb) * kpad * ncol;
   for (int k0 = 0; k0 < npair; k0 += 16) {
     for (int e = tid; e < 32 * 16; e += kGemmThreads) {
       const int r = e >> 4, kk = e & 15, i = rt * 32 + r;
@@ -364,7 +378,9 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_fwd64_kernel(const double* 
     }
     for (int e = tid; e < 16 * 64; e += kGemmThreads) {
       const int kk = e >> 6, c = e & 63;
-      Bs[kk][c] = (k0 + kk < npair && c0 + c < ncol) ? Xm[(int64_t)(k0 + kk) * ncol + c0 + c] : 0.0;
+      const int64_t cc = VEC && half == 1 ? (c0 + c) ^ 3 : c0 + c;
+      const double sg = VEC ? (half == 0 ? vsign0(c0 + c) : vsign1(c0 + c, true)) : 1.0;
+      Bs[kk][c] = (k0 + kk < npair && c0 + c < ncol) ? sg * Xm[(int64_t)(k0 + kk) * ncol + cc] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -378,6 +394,7 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_fwd64_kernel(const double* 
       }
     }
     __syncthreads();
+  }
   }
   const int64_t row0 = p_off[m] / kpad + (p ? pad0 : 0);
 #pragma unroll
@@ -394,25 +411,30 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_fwd64_kernel(const double* 
 
 // Inverse: X[m][p][k][col] = sum_{i < nl[m][p]} P[m][p][i][k] aT[row(m, p, i)][col]
 // for ring pairs k < npair; blockIdx.x = (m * 2 + p) * nkt + ring-pair tile,
-// blockIdx.y = column tile.
+// blockIdx.y = column tile.  VEC: the sum runs over the D0 table of the
+// opposite parity against that parity's coefficients, then over the D1 table
+// (at P + vstride) of the same parity, with the column map and signs above.
+template <bool VEC>
 __global__ __launch_bounds__(kGemmThreads) void gemm_inv64_kernel(const double* __restrict__ aT,
                                                                   double* __restrict__ X,
                                                                   const double* __restrict__ P,
                                                                   const int64_t* __restrict__ p_off,
                                                                   const int* __restrict__ nl, int nkt, int kpad,
-                                                                  int npair, int64_t ncol) {
+                                                                  int npair, int64_t ncol, int64_t vstride) {
   __shared__ double As[16][33];
   __shared__ double Bs[16][65];
   const int mp = blockIdx.x / nkt, kt = blockIdx.x % nkt;
   const int m = mp >> 1, p = mp & 1;
   const int64_t c0 = (int64_t)blockIdx.y * 64;
-  const int nlp = nl[2 * m + p];
   const int pad0 = (nl[2 * m] + kRowPad - 1) / kRowPad * kRowPad;
-  const double* Pm = P + p_off[m] + (p ? (int64_t)pad0 * kpad : 0);
-  const int64_t row0 = p_off[m] / kpad + (p ? pad0 : 0);
   const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
   const int k0 = kt * 32;
   double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+  for (int half = 0; half < (VEC ? 2 : 1); ++half) {
+  const int tp = VEC && half == 0 ? 1 - p : p;  // table and coefficient parity
+  const int nlp = nl[2 * m + tp];
+  const double* Pm = P + half * vstride + p_off[m] + (tp ? (int64_t)pad0 * kpad : 0);
+  const int64_t row0 = p_off[m] / kpad + (tp ? pad0 : 0);
   for (int i0 = 0; i0 < nlp; i0 += 16) {
     for (int e = tid; e < 16 * 32; e += kGemmThreads) {
       const int ii = e >> 5, r = e & 31, k = k0 + r;
@@ -420,7 +442,9 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_inv64_kernel(const double* 
     }
     for (int e = tid; e < 16 * 64; e += kGemmThreads) {
       const int ii = e >> 6, c = e & 63;
-      Bs[ii][c] = (i0 + ii < nlp && c0 + c < ncol) ? aT[(row0 + i0 + ii) * ncol + c0 + c] : 0.0;
+      const int64_t cc = VEC && half == 1 ? (c0 + c) ^ 3 : c0 + c;
+      const double sg = VEC ? (half == 0 ? vsign0(c0 + c) : vsign1(c0 + c, false)) : 1.0;
+      Bs[ii][c] = (i0 + ii < nlp && c0 + c < ncol) ? sg * aT[(row0 + i0 + ii) * ncol + cc] : 0.0;
     }
     __syncthreads();
 #pragma unroll
@@ -434,6 +458,7 @@ __global__ __launch_bounds__(kGemmThreads) void gemm_inv64_kernel(const double* 
       }
     }
     __syncthreads();
+  }
   }
   const int64_t pstride = (int64_t)kpad * ncol;
   double* Xm = X + ((int64_t)m * 2 + p) * pstride;
@@ -456,11 +481,13 @@ __device__ __forceinline__ int64_t at_row64(const int64_t* p_off, const int* nl,
 }
 
 // a[b][l][m] (complex128) from aT; zero for m > l.  One thread per (b, l, m).
+// lnorm: scale by 1 / (l (l + 1)) (1 at l = 0), the vector transform's
+// normalisation (hpx_sht.py:323-328).
 __global__ __launch_bounds__(256) void coeff_to_user64_kernel(const double* __restrict__ aT,
                                                               double* __restrict__ alm,
                                                               const int64_t* __restrict__ p_off,
                                                               const int* __restrict__ nl, int kpad, int lmax,
-                                                              int mmax, int64_t ncol, int64_t nbatch) {
+                                                              int mmax, int64_t ncol, int64_t nbatch, int lnorm) {
   const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int64_t total = nbatch * lmax * (int64_t)mmax;
   if (e >= total) return;
@@ -470,7 +497,8 @@ __global__ __launch_bounds__(256) void coeff_to_user64_kernel(const double* __re
   double2 v = make_double2(0.0, 0.0);
   if (m <= l) {
     const double* src = aT + at_row64(p_off, nl, kpad, m, l) * ncol + 2 * b;
-    v = make_double2(src[0], src[1]);
+    const double sc = (lnorm && l > 0) ? 1.0 / ((double)l * (l + 1)) : 1.0;
+    v = make_double2(sc * src[0], sc * src[1]);
   }
   reinterpret_cast<double2*>(alm)[e] = v;
 }
@@ -560,14 +588,19 @@ int launch_legendre_fwd64(const ShtPlan& p, double* ws, double* alm, int64_t nba
   const int64_t nct = (ncol + 63) / 64;
   if (p.n_items64 > 0) {
     if (nct > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-    gemm_fwd64_kernel<<<dim3((unsigned)p.n_items64, (unsigned)nct), kGemmThreads, 0, s>>>(
-        ws, aT, p.d_p64, p.d_p_off, p.d_nl, p.d_items64, p.kpad, p.npair, ncol);
+    const dim3 g((unsigned)p.n_items64, (unsigned)nct);
+    if (p.vector)
+      gemm_fwd64_kernel<true><<<g, kGemmThreads, 0, s>>>(ws, aT, p.d_p64, p.d_p_off, p.d_nl, p.d_items64, p.kpad,
+                                                          p.npair, ncol, p.vstride);
+    else
+      gemm_fwd64_kernel<false><<<g, kGemmThreads, 0, s>>>(ws, aT, p.d_p64, p.d_p_off, p.d_nl, p.d_items64, p.kpad,
+                                                           p.npair, ncol, 0);
     HPX_LAUNCH_CHECK();
   }
   const int64_t total = nbatch * p.lmax * (int64_t)p.mmax;
   if ((total + 255) / 256 > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   coeff_to_user64_kernel<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(aT, alm, p.d_p_off, p.d_nl, p.kpad, p.lmax,
-                                                                        p.mmax, ncol, nbatch);
+                                                                        p.mmax, ncol, nbatch, p.vector);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
@@ -584,15 +617,22 @@ int launch_legendre_inv64(const ShtPlan& p, const double* alm, double* ws, int64
   const int64_t nx = (int64_t)p.mmax * 2 * nkt;
   const int64_t nct = (ncol + 63) / 64;
   if (nx > 0x7fffffff || nct > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  gemm_inv64_kernel<<<dim3((unsigned)nx, (unsigned)nct), kGemmThreads, 0, s>>>(aT, ws, p.d_p64, p.d_p_off, p.d_nl, nkt,
-                                                                               p.kpad, p.npair, ncol);
+  const dim3 g((unsigned)nx, (unsigned)nct);
+  if (p.vector)
+    gemm_inv64_kernel<true><<<g, kGemmThreads, 0, s>>>(aT, ws, p.d_p64, p.d_p_off, p.d_nl, nkt, p.kpad, p.npair, ncol,
+                                                       p.vstride);
+  else
+    gemm_inv64_kernel<false><<<g, kGemmThreads, 0, s>>>(aT, ws, p.d_p64, p.d_p_off, p.d_nl, nkt, p.kpad, p.npair,
+                                                        ncol, 0);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
 
-int check64(const ShtPlan* p, int64_t nbatch, int64_t ws_bytes, bool need_table) {
+// nbatch: scalar maps (2 per field of a vector call)
+int check64(const ShtPlan* p, int64_t nbatch, int64_t ws_bytes, bool need_table, bool vector_call = false) {
   if (p == nullptr) HPX_FAIL(HPX_ERR_ARG, "null plan");
-  if (p->vector) HPX_FAIL(HPX_ERR_UNSUPPORTED, "FP64 transforms take scalar plans");
+  if (p->vector && !vector_call) HPX_FAIL(HPX_ERR_ARG, "plan holds vector-transform tables (use hpx_vsht_*_f64)");
+  if (!p->vector && vector_call) HPX_FAIL(HPX_ERR_ARG, "plan holds the scalar table (use a vector plan)");
   if (!p->f64) HPX_FAIL(HPX_ERR_ARG, "plan has no FP64 tables: call hpx_sht_plan_enable_f64 first");
   if (need_table && p->lmax < 1) HPX_FAIL(HPX_ERR_ARG, "plan was built with lmax = 0 (ring FFTs only)");
   if (nbatch < 0) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 0");
@@ -620,13 +660,13 @@ ShtPlan* P64(const hpx_sht_plan* h) { return reinterpret_cast<ShtPlan*>(const_ca
 extern "C" int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream) {
   ShtPlan* p = P64(plan);
   if (p == nullptr) HPX_FAIL(HPX_ERR_ARG, "null plan");
-  if (p->vector) HPX_FAIL(HPX_ERR_UNSUPPORTED, "FP64 transforms take scalar plans");
   std::lock_guard<std::mutex> g(p->tune_mu);
   if (p->f64) return HPX_OK;
   if (p->max_M > kMaxM64)
     HPX_FAIL(HPX_ERR_UNSUPPORTED, "FP64 ring FFTs are limited to %d points (this plan needs %d)", kMaxM64, p->max_M);
   const int mmax = p->mmax;
-  const int64_t table = std::max<int64_t>(p->p_off[mmax], 1);
+  // scalar plans: P; vector plans: D0 and D1 (D1 at vstride)
+  const int64_t table = std::max<int64_t>((p->vector ? 2 : 1) * p->p_off[mmax], 1);
   {
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && (uint64_t)(table * sizeof(double)) > (uint64_t)free_b)
@@ -710,8 +750,12 @@ extern "C" int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream) {
     return fail(HPX_ERR_HIP);
   }
   if (mmax > 0 && p->lmax > 0) {
-    legendre_table64_kernel<<<dim3((unsigned)((p->kpad + 255) / 256), (unsigned)mmax), 256, 0, stream>>>(
-        d_p, p->d_p_off, p->d_nl, d_ct, p->npair, p->kpad, p->lmax);
+    if (p->vector) {
+      if (launch_vector_table64(*p, d_p, d_ct, stream) != HPX_OK) return fail(HPX_ERR_HIP);
+    } else {
+      legendre_table64_kernel<<<dim3((unsigned)((p->kpad + 255) / 256), (unsigned)mmax), 256, 0, stream>>>(
+          d_p, p->d_p_off, p->d_nl, d_ct, p->npair, p->kpad, p->lmax);
+    }
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream) != hipSuccess) {
       set_error("FP64 Legendre table generation failed");
       return fail(HPX_ERR_HIP);
@@ -800,4 +844,32 @@ extern "C" int hpx_ring_irfft_f64(const hpx_sht_plan* plan, const double* ftm, d
   int rc = check64(pl, nbatch, -1, false);
   if (rc != HPX_OK || nbatch == 0) return rc;
   return launch_fft64(*pl, false, ftm, f, nbatch, 0, 1.0, 0, 1, stream);
+}
+
+extern "C" int64_t hpx_vsht_workspace_bytes_f64(const hpx_sht_plan* plan, int64_t nbatch) {
+  if (plan == nullptr || nbatch < 0 || nbatch > (INT64_MAX >> 3)) return -1;
+  return workspace_doubles(*P64(plan), 2 * nbatch) * (int64_t)sizeof(double);
+}
+
+extern "C" int hpx_vsht_forward_f64(const hpx_sht_plan* plan, const double* f, double* alm, int64_t nbatch,
+                                    void* workspace, int64_t workspace_bytes, hipStream_t stream) {
+  ShtPlan* pl = P64(plan);
+  if (nbatch < 0 || nbatch > (INT64_MAX >> 3)) HPX_FAIL(HPX_ERR_ARG, "bad nbatch");
+  int rc = check64(pl, 2 * nbatch, workspace_bytes, true, true);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  // the two components of a field are consecutive maps: 2 nbatch ring analyses
+  double* ws = static_cast<double*>(workspace);
+  if ((rc = launch_fft64(*pl, true, f, ws, 2 * nbatch, 1, 1.0, 0, 0, stream)) != HPX_OK) return rc;
+  return launch_legendre_fwd64(*pl, ws, alm, 2 * nbatch, stream);
+}
+
+extern "C" int hpx_vsht_inverse_f64(const hpx_sht_plan* plan, const double* alm, double* f, int64_t nbatch,
+                                    void* workspace, int64_t workspace_bytes, hipStream_t stream) {
+  ShtPlan* pl = P64(plan);
+  if (nbatch < 0 || nbatch > (INT64_MAX >> 3)) HPX_FAIL(HPX_ERR_ARG, "bad nbatch");
+  int rc = check64(pl, 2 * nbatch, workspace_bytes, true, true);
+  if (rc != HPX_OK || nbatch == 0) return rc;
+  double* ws = static_cast<double*>(workspace);
+  if ((rc = launch_legendre_inv64(*pl, alm, ws, 2 * nbatch, stream)) != HPX_OK) return rc;
+  return launch_fft64(*pl, false, ws, f, 2 * nbatch, 0, 1.0, 0, 0, stream);
 }

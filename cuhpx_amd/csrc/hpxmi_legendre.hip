@@ -197,8 +197,10 @@ struct ScaledOrder {
 // from): HPX_NORM_ORTHO leaves them; any other scales both by sqrt(4 pi),
 // and HPX_NORM_SCHMIDT by (2 L + 1)^(-1/2) at the degree L the row reads
 // (L = l for D0, l + 1 for D1); inverse tables take the reciprocals.
+// T: float (the FP32 paths) or double (the FP64 vector transform).
+template <typename T>
 __global__ __launch_bounds__(kThreads) void vector_table_kernel(
-    float* __restrict__ D, int64_t vstride, const int64_t* __restrict__ p_off, const int* __restrict__ nl,
+    T* __restrict__ D, int64_t vstride, const int64_t* __restrict__ p_off, const int* __restrict__ nl,
     const double* __restrict__ ct, int npair, int kpad, int lmax, int csphase, int norm, int inverse) {
   const int k = blockIdx.x * kThreads + threadIdx.x;
   const int m = blockIdx.y;
@@ -206,19 +208,19 @@ __global__ __launch_bounds__(kThreads) void vector_table_kernel(
   const int nl0 = nl[2 * m], nl1 = nl[2 * m + 1];
   const int pad0 = (nl0 + kRowPad - 1) / kRowPad * kRowPad;
   const int pad1 = (nl1 + kRowPad - 1) / kRowPad * kRowPad;
-  float* T0[2] = {D + p_off[m], D + p_off[m] + (int64_t)pad0 * kpad};
-  float* T1[2] = {T0[0] + vstride, T0[1] + vstride};
-  for (int i = nl0; i < pad0; ++i) T0[0][(int64_t)i * kpad + k] = T1[0][(int64_t)i * kpad + k] = 0.f;
-  for (int i = nl1; i < pad1; ++i) T0[1][(int64_t)i * kpad + k] = T1[1][(int64_t)i * kpad + k] = 0.f;
+  T* T0[2] = {D + p_off[m], D + p_off[m] + (int64_t)pad0 * kpad};
+  T* T1[2] = {T0[0] + vstride, T0[1] + vstride};
+  for (int i = nl0; i < pad0; ++i) T0[0][(int64_t)i * kpad + k] = T1[0][(int64_t)i * kpad + k] = T(0);
+  for (int i = nl1; i < pad1; ++i) T0[1][(int64_t)i * kpad + k] = T1[1][(int64_t)i * kpad + k] = T(0);
   if (k >= npair) {
-    for (int i = 0; i < nl0; ++i) T0[0][(int64_t)i * kpad + k] = T1[0][(int64_t)i * kpad + k] = 0.f;
-    for (int i = 0; i < nl1; ++i) T0[1][(int64_t)i * kpad + k] = T1[1][(int64_t)i * kpad + k] = 0.f;
+    for (int i = 0; i < nl0; ++i) T0[0][(int64_t)i * kpad + k] = T1[0][(int64_t)i * kpad + k] = T(0);
+    for (int i = 0; i < nl1; ++i) T0[1][(int64_t)i * kpad + k] = T1[1][(int64_t)i * kpad + k] = T(0);
     return;
   }
   if (m >= lmax) return;
   const double x = ct[k];
   const double s = sqrt((1.0 - x) * (1.0 + x));
-  const float sign = (csphase && (m & 1)) ? -1.f : 1.f;
+  const T sign = (csphase && (m & 1)) ? T(-1) : T(1);
   const double dm = m;
   ScaledOrder lo, hi;  // orders m - 1 and m + 1
   hi.init(m + 1, x, s);
@@ -251,8 +253,8 @@ __global__ __launch_bounds__(kThreads) void vector_table_kernel(
     }
     const int par = (l - m) & 1;
     const int64_t off = (int64_t)((l - m) >> 1) * kpad + k;
-    T0[par][off] = sign * (float)d0;
-    T1[par][off] = sign * (float)d1;
+    T0[par][off] = sign * (T)d0;
+    T1[par][off] = sign * (T)d1;
   }
 }
 
@@ -1039,7 +1041,7 @@ __global__ __launch_bounds__(kThreads) void user_to_coeff_kernel(
 int launch_legendre_table(ShtPlan& p, const double* d_ct, hipStream_t s) {
   dim3 grid((p.kpad + kThreads - 1) / kThreads, p.mmax);
   if (p.vector) {
-    vector_table_kernel<<<grid, kThreads, 0, s>>>(p.d_p, p.vstride, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
+    vector_table_kernel<float><<<grid, kThreads, 0, s>>>(p.d_p, p.vstride, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
                                                   p.lmax, p.csphase, p.vnorm, p.vinverse);
   } else {
     checkpoint_kernel<<<grid, kThreads, 0, s>>>(p.d_ck, p.d_seg_off, d_ct, p.npair, p.kpad, p.lmax);
@@ -1060,6 +1062,14 @@ int launch_legendre_table(ShtPlan& p, const double* d_ct, hipStream_t s) {
       }
     }
   }
+  return HPX_OK;
+}
+
+int launch_vector_table64(const ShtPlan& p, double* D, const double* d_ct, hipStream_t s) {
+  dim3 grid((p.kpad + kThreads - 1) / kThreads, p.mmax);
+  vector_table_kernel<double><<<grid, kThreads, 0, s>>>(D, p.vstride, p.d_p_off, p.d_nl, d_ct, p.npair, p.kpad,
+                                                         p.lmax, p.csphase, p.vnorm, p.vinverse);
+  HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
 

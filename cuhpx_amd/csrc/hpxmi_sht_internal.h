@@ -204,5 +204,8 @@ int launch_legendre_fwd(const ShtPlan& p, int precision, float* ws, float* alm,
 int launch_legendre_inv(const ShtPlan& p, int precision, const float* alm, float* ws,
                         int64_t nbatch, hipStream_t s);
 int launch_legendre_table(ShtPlan& p, const double* theta_north, hipStream_t s);
+// The two vector tables in FP64 (layout of the FP32 vector tables, D1 at
+// D + vstride), for the FP64 vector transform.
+int launch_vector_table64(const ShtPlan& p, double* D, const double* cos_theta, hipStream_t s);
 
 }  // namespace hpxmi

@@ -162,7 +162,7 @@ class ShtPlan:
         """Workspace for nbatch maps (scalar plan) or nbatch vector fields;
         ``f64``: for the FP64 calls."""
         if f64:
-            fn = _lib.lib().hpx_sht_workspace_bytes_f64
+            fn = _lib.lib().hpx_vsht_workspace_bytes_f64 if self.vector else _lib.lib().hpx_sht_workspace_bytes_f64
         else:
             fn = _lib.lib().hpx_vsht_workspace_bytes if self.vector else _lib.lib().hpx_sht_workspace_bytes
         nbytes = int(fn(self._handle, nbatch))
@@ -244,15 +244,27 @@ class ShtPlan:
         return out
 
     def vector_forward(self, f, precision):
-        """f [n, 2, npix] float32 contiguous -> alm [n, 2, lmax, mmax] complex64."""
+        """f [n, 2, npix] float32 (float64) contiguous -> alm [n, 2, lmax, mmax]
+        complex64 (complex128, computed in FP64)."""
         n = f.shape[0]
+        if f.dtype == torch.float64:
+            self.enable_f64()
+            out = torch.empty((n, 2, self.lmax, self.mmax), dtype=torch.complex128, device=f.device)
+            self._call(_lib.lib().hpx_vsht_forward_f64, (), f, out, "vector SHT forward (FP64)", f64=True)
+            return out
         out = torch.empty((n, 2, self.lmax, self.mmax), dtype=torch.complex64, device=f.device)
         self._call(_lib.lib().hpx_vsht_forward, (_lib.PRECISIONS[precision],), f, out, "vector SHT forward")
         return out
 
     def vector_inverse(self, alm, precision):
-        """alm [n, 2, lmax, mmax] complex64 contiguous -> f [n, 2, npix] float32."""
+        """alm [n, 2, lmax, mmax] complex64 (complex128) contiguous -> f [n, 2, npix]
+        float32 (float64, computed in FP64)."""
         n = alm.shape[0]
+        if alm.dtype == torch.complex128:
+            self.enable_f64()
+            out = torch.empty((n, 2, self.npix), dtype=torch.float64, device=alm.device)
+            self._call(_lib.lib().hpx_vsht_inverse_f64, (), alm, out, "vector SHT inverse (FP64)", f64=True)
+            return out
         out = torch.empty((n, 2, self.npix), dtype=torch.float32, device=alm.device)
         self._call(_lib.lib().hpx_vsht_inverse, (_lib.PRECISIONS[precision],), alm, out, "vector SHT inverse")
         return out
@@ -458,8 +470,8 @@ class VectorSHT(_HealpixVectorBase):
     def forward(self, x):
         if torch.is_complex(x):
             raise ValueError("Input tensor must be real.")
-        if x.dtype != torch.float32:
-            raise TypeError(f"VectorSHT computes in float32; got {x.dtype}")
+        if x.dtype not in (torch.float32, torch.float64):
+            raise TypeError(f"VectorSHT takes float32 or float64 fields; got {x.dtype}")
         plan = self.plan_for(x.device)
         if x.dim() < 2 or x.shape[-2] != 2 or x.shape[-1] != plan.npix:
             raise RuntimeError(f"input must be [..., 2, {plan.npix}], got {tuple(x.shape)}")
@@ -486,8 +498,8 @@ class VectoriSHT(_HealpixVectorBase):
     def forward(self, x):
         if not torch.is_complex(x):
             raise ValueError("Input tensor must be complex.")
-        if x.dtype != torch.complex64:
-            raise TypeError(f"VectoriSHT computes in complex64; got {x.dtype}")
+        if x.dtype not in (torch.complex64, torch.complex128):
+            raise TypeError(f"VectoriSHT takes complex64 or complex128 coefficients; got {x.dtype}")
         plan = self.plan_for(x.device)
         if x.dim() < 3 or tuple(x.shape[-3:]) != (2, plan.lmax, plan.mmax):
             raise RuntimeError(f"input must be [..., 2, {plan.lmax}, {plan.mmax}], got {tuple(x.shape)}")

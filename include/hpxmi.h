@@ -194,12 +194,13 @@ int hpx_sht_inverse_adjoint(const hpx_sht_plan* plan, int precision, int mode,
  * (cuhpx/hpx_sht.py:557, 599).  These calls are that path: FP64 ring FFTs
  * (Bluestein tables in FP64) and FP64 Legendre GEMMs on an FP64 table.
  *
- * hpx_sht_plan_enable_f64 builds a scalar plan's FP64 state (the FP64
- * Legendre table in HBM: 8 bytes per (m, l, ring pair) of the folded
- * triangle, 0.6 GB at nside 256, lmax = mmax = 513; Bluestein tables,
- * twiddles, phases); it allocates, synchronises the stream, and is a no-op
- * on a plan that has it.  The _f64 calls below return HPX_ERR_ARG on a plan
- * without it.  Rings longer than 8192 points (nside > 1024) are refused. */
+ * hpx_sht_plan_enable_f64 builds a plan's FP64 state (the FP64 Legendre
+ * table in HBM: 8 bytes per (m, l, ring pair) of the folded triangle, 0.6 GB
+ * at nside 256, lmax = mmax = 513, twice that for the two tables of a vector
+ * plan; Bluestein tables, twiddles, phases); it allocates, synchronises the
+ * stream, and is a no-op on a plan that has it.  The _f64 calls below return
+ * HPX_ERR_ARG on a plan without it.  Rings longer than 8192 points
+ * (nside > 1024) are refused. */
 int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream);
 
 /* Workspace bytes of an FP64 call on nbatch maps. */
@@ -227,6 +228,19 @@ int hpx_ring_rfft_f64(const hpx_sht_plan* plan, const double* f, double* ftm, in
                       hipStream_t stream);
 int hpx_ring_irfft_f64(const hpx_sht_plan* plan, const double* ftm, double* f, int64_t nbatch,
                        hipStream_t stream);
+
+/* hpx_vsht_forward / hpx_vsht_inverse in FP64 on a vector plan (after
+ * hpx_sht_plan_enable_f64): f float64 [nbatch, 2, npix], alm complex128
+ * [nbatch, 2, lmax, mmax]; the reference's VectorSHT / VectoriSHT on float64
+ * input (their einsums cast the tables to the input's dtype, hpx_sht.py:
+ * 351-372, 413-432).  Workspace: hpx_vsht_workspace_bytes_f64. */
+int64_t hpx_vsht_workspace_bytes_f64(const hpx_sht_plan* plan, int64_t nbatch);
+int hpx_vsht_forward_f64(const hpx_sht_plan* plan, const double* f, double* alm,
+                         int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                         hipStream_t stream);
+int hpx_vsht_inverse_f64(const hpx_sht_plan* plan, const double* alm, double* f,
+                         int64_t nbatch, void* workspace, int64_t workspace_bytes,
+                         hipStream_t stream);
 
 /* ------------------------------------------------------------------ */
 /* Vector (spheroidal / toroidal) SHT                                  */

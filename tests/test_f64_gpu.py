@@ -139,3 +139,27 @@ def test_f64_regridding_round_trip_matches_oracle():
     ref_eq = oe.inverse(oh.forward(f))
     assert rel_l2(eq.cpu().numpy().reshape(2, -1), ref_eq) < TOL64
     assert rel_l2(back, oh.inverse(oe.forward(ref_eq))) < TOL64
+
+
+@pytest.mark.parametrize("nside,lmax,mmax,norm", [(4, 9, 7, "ortho"), (8, 31, 17, "ortho"), (16, 33, 33, "ortho"),
+                                                  (8, 17, 13, "schmidt")])
+def test_f64_vector_transforms_match_oracle(nside, lmax, mmax, norm):
+    """VectorSHT / VectoriSHT on float64 / complex128 compute in FP64, against
+    the FP64 vector oracle."""
+    from oracle.vector import VectorSHTOracle
+    o = VectorSHTOracle.healpix(nside, lmax, mmax, norm=norm)
+    rng = np.random.default_rng(49)
+    f = rng.standard_normal((2, 2, o.grid.npix))
+    fwd = cuhpx_amd.VectorSHT(nside, lmax=lmax, mmax=mmax, norm=norm)
+    inv = cuhpx_amd.VectoriSHT(nside, lmax=lmax, mmax=mmax, norm=norm)
+    c = fwd(torch.from_numpy(f).cuda())
+    assert c.dtype == torch.complex128 and c.shape == (2, 2, lmax, mmax)
+    ef = rel_l2(c.cpu().numpy(), o.forward(f))
+    a = rng.standard_normal((2, 2, lmax, mmax)) + 1j * rng.standard_normal((2, 2, lmax, mmax))
+    a[..., np.arange(lmax)[:, None] < np.arange(mmax)[None, :]] = 0
+    a[..., 0] = a[..., 0].real
+    g = inv(torch.from_numpy(a).cuda())
+    assert g.dtype == torch.float64 and g.shape == (2, 2, o.grid.npix)
+    ei = rel_l2(g.cpu().numpy(), o.inverse(a))
+    print(f"[f64 vector nside={nside} L={lmax} M={mmax} {norm}] forward {ef:.2e} inverse {ei:.2e}", flush=True)
+    assert ef < TOL64 and ei < TOL64

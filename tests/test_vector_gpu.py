@@ -119,7 +119,7 @@ def test_vector_input_checks():
     with pytest.raises(ValueError):
         fwd(torch.zeros((2, npix), dtype=torch.complex64, device="cuda"))
     with pytest.raises(TypeError):
-        fwd(torch.zeros((2, npix), dtype=torch.float64, device="cuda"))
+        fwd(torch.zeros((2, npix), dtype=torch.float16, device="cuda"))
     with pytest.raises(RuntimeError):
         fwd(torch.zeros((3, npix), device="cuda"))
     with pytest.raises(ValueError):
