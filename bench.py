@@ -70,15 +70,21 @@ def band_limited_alm(lmax, mmax, nfield, seed):
 
 
 def algorithmic_figures():
-    """Per-map algorithmic work, forward + inverse (one direction = half)."""
+    """Per-map algorithmic work, forward + inverse (one direction = half).
+    The Legendre GEMMs generate P_lm from FP64 checkpoints (the pair
+    (P_l0, P_l0+1) every 16 degrees per ring pair, 8 bytes each), which they
+    read once per direction in place of the 4 * N_lm * n_half bytes of a
+    stored table."""
     nring = 4 * NSIDE - 1
     npix = 12 * NSIDE * NSIDE
     n_half = 2 * NSIDE
     n_lm = sum(LMAX - m for m in range(MMAX))
+    kpad = (n_half + 31) // 32 * 32
+    nseg = sum((LMAX - m + 15) // 16 for m in range(MMAX) if m < LMAX)
     return {
         "legendre_flop_per_map_per_dir": 4.0 * n_lm * n_half,
         "legendre_io_bytes_per_map_per_dir": 8.0 * nring * MMAX + 8.0 * n_lm,
-        "legendre_table_bytes_per_dir": 4.0 * n_lm * n_half,
+        "legendre_checkpoint_bytes_per_dir": 8.0 * nseg * kpad,
         "fft_bytes_per_map_per_dir": 4.0 * npix + 8.0 * nring * MMAX,
         "n_lm": n_lm,
     }
@@ -91,7 +97,7 @@ MFMA_PEAK = {"fp32": PEAK["fp32_TFLOPs"], "bf16x3": PEAK["bf16_TFLOPs"] / 3, "bf
 
 def stage_roofline(name, ms, precision):
     """Roofline of one stage of the batch: algorithmic bytes (maps, spectra,
-    coefficients, Legendre table) and FP32-equivalent flops; the stage is
+    coefficients, Legendre checkpoints) and FP32-equivalent flops; the stage is
     bound by HBM when its arithmetic intensity is below the ridge of the
     precision's MFMA peak, else by MFMA."""
     fig = algorithmic_figures()
@@ -102,7 +108,7 @@ def stage_roofline(name, ms, precision):
         return {"kernel": name, "bound": "hbm", "achieved": gbs, "peak": PEAK["hbm_GBs"], "unit": "GB/s",
                 "frac": gbs / PEAK["hbm_GBs"], "algorithmic_bytes": nbytes}
     flop = fig["legendre_flop_per_map_per_dir"] * BATCH
-    nbytes = fig["legendre_io_bytes_per_map_per_dir"] * BATCH + fig["legendre_table_bytes_per_dir"]
+    nbytes = fig["legendre_io_bytes_per_map_per_dir"] * BATCH + fig["legendre_checkpoint_bytes_per_dir"]
     peak = MFMA_PEAK[precision]
     ridge = peak * 1e12 / (PEAK["hbm_GBs"] * 1e9)
     ai = flop / nbytes
