@@ -419,9 +419,10 @@ Side flat_side(int origin, int cw) {
 
 // want: work-groups to aim for (tiles x row chunks); more chunks mean fewer
 // rows per work-group and the tile's index arithmetic done more often.
+// rows_per_wg > 0 instead fixes the rows per work-group.
 template <typename T, int TBMAX, int PT, bool NTL, bool NTS>
 int launch_tiles(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Side dst,
-                 hipStream_t stream, int64_t want) {
+                 hipStream_t stream, int64_t want, int rows_per_wg = 0) {
   constexpr int kThreads = TBMAX * TBMAX / PT;
   int order = 0;
   while ((int64_t(1) << order) < nside) ++order;
@@ -430,8 +431,9 @@ int launch_tiles(const void* in, void* out, int64_t nside, int64_t nrows, Side s
   while ((1 << log_tb) < tb) ++log_tb;
   const int tiles_per_face = (int)((nside / tb) * (nside / tb));
   const int ntiles = 12 * tiles_per_face;
-  int64_t chunks = (want + ntiles - 1) / ntiles;
+  int64_t chunks = rows_per_wg > 0 ? (nrows + rows_per_wg - 1) / rows_per_wg : (want + ntiles - 1) / ntiles;
   if (chunks > nrows) chunks = nrows;
+  if (chunks > 65535) chunks = 65535;  // grid y limit
   if (chunks < 1) chunks = 1;
   const int rows_per_chunk = (int)((nrows + chunks - 1) / chunks);
   chunks = (nrows + rows_per_chunk - 1) / rows_per_chunk;
@@ -463,15 +465,16 @@ int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Si
     }
   }
   // NEST <-> flat and flat -> flat: 64 x 64 tiles, 4 pixels per thread (1024
-  // threads) and about 16 work-groups per tile and face region at nside 1024
-  // (4 rows each of a 64-row batch): more, shorter-lived work-groups keep
+  // threads) and 4 rows per work-group: more, shorter-lived work-groups keep
   // more loads in flight (nside 1024, [64, npix] float32: nest2flat
   // 5.2 -> 5.8 TB/s, flat2flat 4.9 -> 5.4 TB/s, against 2048 work-groups of
-  // 256 threads looping over all 64 rows).  The RING kinds gained nothing
-  // from either change.
+  // 256 threads looping over all 64 rows).  One row per work-group repeats
+  // the tile's index arithmetic too often (nside 512, 64 rows: 2.9 TB/s
+  // against 5.6 with 4 rows; 2 rows 4.4-4.6, 8 rows 5.4-5.5).  The RING
+  // kinds gained nothing from either change.
   if (!ring && sizeof(T) <= 4) {
-    if (nest_flat) return launch_tiles<T, 64, 4, true, true>(in, out, nside, nrows, src, dst, stream, 49152);
-    return launch_tiles<T, 64, 4, false, false>(in, out, nside, nrows, src, dst, stream, 49152);
+    if (nest_flat) return launch_tiles<T, 64, 4, true, true>(in, out, nside, nrows, src, dst, stream, 0, 4);
+    return launch_tiles<T, 64, 4, false, false>(in, out, nside, nrows, src, dst, stream, 0, 4);
   }
   if (nest_flat) return launch_tiles<T, 64, 16, true, true>(in, out, nside, nrows, src, dst, stream, 2048);
   return launch_tiles<T, 64, 16, false, false>(in, out, nside, nrows, src, dst, stream, 2048);
