@@ -419,10 +419,11 @@ Side flat_side(int origin, int cw) {
 
 // want: work-groups to aim for (tiles x row chunks); more chunks mean fewer
 // rows per work-group and the tile's index arithmetic done more often.
-// rows_per_wg > 0 instead fixes the rows per work-group.
+// rows_per_wg > 0 instead fixes the rows per work-group; min_rows keeps at
+// least that many rows per work-group (fewer chunks) where want asks for more.
 template <typename T, int TBMAX, int PT, bool NTL, bool NTS>
 int launch_tiles(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Side dst,
-                 hipStream_t stream, int64_t want, int rows_per_wg = 0) {
+                 hipStream_t stream, int64_t want, int rows_per_wg = 0, int min_rows = 1) {
   constexpr int kThreads = TBMAX * TBMAX / PT;
   int order = 0;
   while ((int64_t(1) << order) < nside) ++order;
@@ -432,6 +433,7 @@ int launch_tiles(const void* in, void* out, int64_t nside, int64_t nrows, Side s
   const int tiles_per_face = (int)((nside / tb) * (nside / tb));
   const int ntiles = 12 * tiles_per_face;
   int64_t chunks = rows_per_wg > 0 ? (nrows + rows_per_wg - 1) / rows_per_wg : (want + ntiles - 1) / ntiles;
+  if (min_rows > 1 && chunks > nrows / min_rows) chunks = nrows / min_rows;
   if (chunks > nrows) chunks = nrows;
   if (chunks > 65535) chunks = 65535;  // grid y limit
   if (chunks < 1) chunks = 1;
@@ -458,10 +460,15 @@ int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Si
     if (ring && nside >= 128) {
       // RING -> NEST / flat: the destination streams whole lines, stored
       // non-temporally (ring2nest 1.33 -> 1.25 ms at nside 1024, 64 rows);
-      // non-temporal loads of a NEST / flat source measured slower
+      // non-temporal loads of a NEST / flat source measured slower.  Unlike
+      // the NEST / flat kinds these gain from long row loops: at least 16
+      // rows per work-group (64 rows of [rows, npix] float32, ring2nest /
+      // nest2ring at nside 256: 3.6 / 3.0 TB/s with 4 rows, 4.5 / 3.6 with
+      // 16, 3.7 / 2.8 with 32; nside 128: 1.5 -> 1.7 TB/s; nside >= 512
+      // already ran 16-64 rows per work-group)
       if (src.scheme == kRing)
-        return launch_tiles<T, 128, 16, false, true>(in, out, nside, nrows, src, dst, stream, 768);
-      return launch_tiles<T, 128, 16, false, false>(in, out, nside, nrows, src, dst, stream, 768);
+        return launch_tiles<T, 128, 16, false, true>(in, out, nside, nrows, src, dst, stream, 768, 0, 16);
+      return launch_tiles<T, 128, 16, false, false>(in, out, nside, nrows, src, dst, stream, 768, 0, 16);
     }
   }
   // NEST <-> flat and flat -> flat: 64 x 64 tiles, 4 pixels per thread (1024
