@@ -492,8 +492,10 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
   const int jb = items[item].y;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int col0 = cb * (64 * CT) + wave * (16 * CT);
-  const bool wave_active = col0 < ncol;
+  // a wave whose columns lie beyond the batch loads and multiplies the block's
+  // first columns instead and stores nothing, so that every wave issues the
+  // same loads
+  const int col0 = cb * (64 * CT) + wave * (16 * CT) < ncol ? cb * (64 * CT) + wave * (16 * CT) : 0;
   const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
   const int padp0 = (nlp[0] + kRowPad - 1) / kRowPad * kRowPad;
   const int padp1 = (nlp[1] + kRowPad - 1) / kRowPad * kRowPad;
@@ -525,8 +527,12 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
   const int gk = threadIdx.x & 31;
   const int sbase = VEC ? 0 : g.seg_off[m];
   const int nsg = VEC ? 0 : g.seg_off[m + 1] - sbase;
-  const bool gseg_ok = jb * 8 + gs < nsg;
-  const float2* ckp = VEC ? nullptr : g.ck + (int64_t)(sbase + jb * 8 + gs) * kpad + gk;
+  // this order's checkpoints through a buffer descriptor: a segment past the
+  // order's last lies beyond it and reads zero, with no branch
+  const __amdgpu_buffer_rsrc_t ckr =
+      buffer_rsrc(reinterpret_cast<const float*>(VEC ? nullptr : g.ck + (int64_t)sbase * kpad),
+                  VEC ? 0 : (int64_t)nsg * kpad * 8);
+  const int ckoff = ((jb * 8 + gs) * kpad + gk) * 8;
   if constexpr (!VEC) {
     if (threadIdx.x < 8 * kSegL) {
       const int ts = threadIdx.x / kSegL, tj = threadIdx.x % kSegL;
@@ -543,9 +549,8 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
 
   auto load_a = [&](int kc, float* v) {
     if constexpr (!VEC) {  // the checkpoint and the ring cosine; store_a generates
-      const float2 c = gseg_ok ? ckp[kc] : make_float2(0.f, 0.f);
-      v[0] = c.x;
-      v[1] = c.y;
+      v[0] = buffer_load(ckr, ckoff, kc * 8);
+      v[1] = buffer_load(ckr, ckoff + 4, kc * 8);
       v[2] = g.ct[kc + gk];
       return;
     }
@@ -627,31 +632,34 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
   auto kof = [&](int t) { return (VEC && t >= half ? kpad + (t - half) * kKTile : t * kKTile) + c0 * kKTile; };
 
   // software pipeline over k chunks with two register sets for B and two
-  // LDS buffers for the table tile (one barrier per chunk, no copies)
+  // LDS buffers for the table tile (one barrier per chunk, no copies).  Every
+  // half-iteration issues its loads unconditionally (the last one reloads
+  // the last chunk, and its tile is never read): a load issued on only some
+  // paths makes the compiler wait for every load in flight before the
+  // products of the current chunk, which serialises the prefetch.
   float pa[16];
   float b0[2][CT][8], b1[2][CT][8];
   if (nchunk > 0) {
     load_a(kof(0), pa);
-    if (wave_active) load_b(kof(0), b0);
+    load_b(kof(0), b0);
     store_a(lds[0], pa);
     __syncthreads();
-  }
-  for (int c = 0; c < nchunk; c += 2) {
-    if (c + 1 < nchunk) {
-      load_a(kof(c + 1), pa);
-      if (wave_active) load_b(kof(c + 1), b1);
+    for (int c = 0;; c += 2) {
+      const int c1 = min(c + 1, nchunk - 1);
+      load_a(kof(c1), pa);
+      load_b(kof(c1), b1);
+      mfma_chunk<PREC, CT>(lds[0], b0, acc, lane);
+      store_a(lds[1], pa);
+      __syncthreads();
+      if (c + 1 >= nchunk) break;
+      const int c2 = min(c + 2, nchunk - 1);
+      load_a(kof(c2), pa);
+      load_b(kof(c2), b0);
+      mfma_chunk<PREC, CT>(lds[1], b1, acc, lane);
+      store_a(lds[0], pa);
+      __syncthreads();
+      if (c + 2 >= nchunk) break;
     }
-    if (wave_active) mfma_chunk<PREC, CT>(lds[0], b0, acc, lane);
-    if (c + 1 < nchunk) store_a(lds[1], pa);
-    __syncthreads();
-    if (c + 1 >= nchunk) break;
-    if (c + 2 < nchunk) {
-      load_a(kof(c + 2), pa);
-      if (wave_active) load_b(kof(c + 2), b0);
-    }
-    if (wave_active) mfma_chunk<PREC, CT>(lds[1], b1, acc, lane);
-    if (c + 2 < nchunk) store_a(lds[0], pa);
-    __syncthreads();
   }
 
   // ---- this m's rows of aT (rows row0.. of each parity, this work-group's columns)
@@ -709,8 +717,10 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int rh = wave >> 2;
-  const int col0 = cb * (64 * CT) + (wave & 3) * (16 * CT);
-  const bool wave_active = col0 < ncol;
+  // a wave whose columns lie beyond the batch loads and multiplies the block's
+  // first columns instead and stores nothing (every wave issues the same loads)
+  const int col0 =
+      cb * (64 * CT) + (wave & 3) * (16 * CT) < ncol ? cb * (64 * CT) + (wave & 3) * (16 * CT) : 0;
   const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
   const int padp[2] = {(nlp[0] + kRowPad - 1) / kRowPad * kRowPad, (nlp[1] + kRowPad - 1) / kRowPad * kRowPad};
   const int nmax = nlp[0] > nlp[1] ? nlp[0] : nlp[1];
@@ -753,14 +763,18 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
   const float gx = (!VEC && gkq < kpad) ? g.ct[gkq] : 0.f;
   const int sbase = VEC ? 0 : g.seg_off[m];
   const int nsg = VEC ? 0 : g.seg_off[m + 1] - sbase;
+  // this order's checkpoints through a buffer descriptor: a segment past the
+  // order's last, or a ring pair past the last, reads zero, with no branch
+  const __amdgpu_buffer_rsrc_t ckr =
+      buffer_rsrc(reinterpret_cast<const float*>(VEC ? nullptr : g.ck + (int64_t)sbase * kpad),
+                  VEC ? 0 : (int64_t)nsg * kpad * 8);
+  const int ckoff = gkq < kpad ? (gg * kpad + gkq) * 8 : nsg * kpad * 8;
 
   // table tile: this block's 64 ring pairs, i chunk starting at ic
   auto load_a = [&](int ic, float* v) {
     if constexpr (!VEC) {  // the checkpoint; store_a generates
-      const int seg = ic / 8 + gg;
-      const float2 c = (seg < nsg && gkq < kpad) ? g.ck[(int64_t)(sbase + seg) * kpad + gkq] : make_float2(0.f, 0.f);
-      v[0] = c.x;
-      v[1] = c.y;
+      v[0] = buffer_load(ckr, ckoff, (ic / 8) * kpad * 8);
+      v[1] = buffer_load(ckr, ckoff + 4, (ic / 8) * kpad * 8);
       return;
     }
     const bool half1 = VEC && ic >= kp;
@@ -781,15 +795,21 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
       for (int q = 0; q < 16; ++q) v[q] = 0.f;
     }
   };
-  // ic: the chunk of the tile being written (generation only)
-  auto store_a = [&](Lds& L, const float* v, int ic = 0) {
+  // the recurrence coefficients of the segment of chunk ic (wave-uniform,
+  // scalar loads); a segment past the last one is never used (zero
+  // checkpoint): the last segment's are read instead of running off the table
+  auto load_cf = [&](int ic, float2 (&cf)[kSegL]) {
     if constexpr (!VEC) {
-      // coefficients of a segment past the last one are never used (zero
-      // checkpoint); read the last segment's instead of running off the table
       const int seg = min(ic / 8 + gg, nsg - 1);
       const float2* abp = g.ab + (int64_t)(sbase + seg) * kSegL;
+#pragma unroll
+      for (int j = 2; j < kSegL; ++j) cf[j] = abp[j];
+    }
+  };
+  auto store_a = [&](Lds& L, const float* v, const float2 (&cf)[kSegL]) {
+    if constexpr (!VEC) {
       float w[kSegL];
-      segment_recurrence(w, make_float2(v[0], v[1]), gx, [&](int j) { return abp[j]; });
+      segment_recurrence(w, make_float2(v[0], v[1]), gx, [&](int j) { return cf[j]; });
       const int c8 = 8 * gg;
 #pragma unroll
       for (int p = 0; p < 2; ++p) {
@@ -846,6 +866,17 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
         for (int q = 0; q < 8; ++q)
           bv[p][ct][q] = buffer_load(
               ar, voff, (int)(((int64_t)((pb ? padp[0] : 0) + ii + frag_k_uniform<PREC>(q)) * nc + ct * 16) * 4));
+    }
+  };
+  // Just before the products (the loads have landed by then; writing a
+  // register that a load in flight targets would wait for it): rows at or
+  // beyond the parity's count set to zero, and the vector transform's signs.
+  auto prep_b = [&](int ic, float (&bv)[2][CT][8]) {
+    const bool half1 = VEC && ic >= kp;
+    const int ii = half1 ? ic - kp : ic;
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int pb = VEC && !half1 ? 1 - p : p;
       if (ii + kKTile > nlp[pb]) {
 #pragma unroll
         for (int ct = 0; ct < CT; ++ct)
@@ -865,30 +896,39 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
     }
   };
 
+  // Software pipeline as in the forward kernel: every half-iteration issues
+  // its loads unconditionally (the last reloads the last chunk, whose tile
+  // is never read); the checkpoints and the coefficients of the chunk after
+  // next are requested before the products of this one.
   const int nchunk = (VEC ? 2 : 1) * kp / kKTile;  // 0 for m >= lmax: zero spectra
   if (nchunk > 0) {
     float pa[16];
+    float2 cf[kSegL];
     float b0[2][CT][8], b1[2][CT][8];
     load_a(0, pa);
-    if (wave_active) load_b(0, b0);
-    store_a(lds[0], pa, 0);
+    load_cf(0, cf);
+    load_b(0, b0);
+    store_a(lds[0], pa, cf);
     __syncthreads();
-    for (int c = 0; c < nchunk; c += 2) {
-      if (c + 1 < nchunk) {
-        load_a((c + 1) * kKTile, pa);
-        if (wave_active) load_b((c + 1) * kKTile, b1);
-      }
-      if (wave_active) mfma_chunk<PREC, CT>(lds[0], b0, acc, lane, rh * 64);
-      if (c + 1 < nchunk) store_a(lds[1], pa, (c + 1) * kKTile);
+    for (int c = 0;; c += 2) {
+      const int c1 = min(c + 1, nchunk - 1) * kKTile;
+      load_a(c1, pa);
+      load_cf(c1, cf);
+      load_b(c1, b1);
+      prep_b(c * kKTile, b0);
+      mfma_chunk<PREC, CT>(lds[0], b0, acc, lane, rh * 64);
+      store_a(lds[1], pa, cf);
       __syncthreads();
       if (c + 1 >= nchunk) break;
-      if (c + 2 < nchunk) {
-        load_a((c + 2) * kKTile, pa);
-        if (wave_active) load_b((c + 2) * kKTile, b0);
-      }
-      if (wave_active) mfma_chunk<PREC, CT>(lds[1], b1, acc, lane, rh * 64);
-      if (c + 2 < nchunk) store_a(lds[0], pa, (c + 2) * kKTile);
+      const int c2 = min(c + 2, nchunk - 1) * kKTile;
+      load_a(c2, pa);
+      load_cf(c2, cf);
+      load_b(c2, b0);
+      prep_b((c + 1) * kKTile, b1);
+      mfma_chunk<PREC, CT>(lds[1], b1, acc, lane, rh * 64);
+      store_a(lds[0], pa, cf);
       __syncthreads();
+      if (c + 2 >= nchunk) break;
     }
   }
 
