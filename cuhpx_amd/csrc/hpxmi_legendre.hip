@@ -54,6 +54,20 @@ __device__ __forceinline__ void split(float x, __bf16& hi, __bf16& lo) {
   lo = (__bf16)(x - (float)hi);
 }
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// split() of two values at once, as packed BF16 pairs (value a in the low
+// half): hi = RNE(a, b) by one v_cvt_pk_bf16_f32, its halves widened back by
+// a shift and a mask, lo = RNE(a - hi_a, b - hi_b) by a second conversion:
+// 3 VALU per value where the compiler makes 4 of split() (bit-identical).
+__device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
+  const floatx2 v = {a, b};
+  hi = __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
+  const floatx2 r = {a - __builtin_bit_cast(float, hi << 16), b - __builtin_bit_cast(float, hi & 0xffff0000u)};
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2));
+}
+
 // ---------------------------------------------------------------------------
 // Table generation (FP64, scaled recurrence)
 // ---------------------------------------------------------------------------
@@ -302,16 +316,15 @@ using FwdLds = GemmLds<PREC, kRowsPerParity>;
 // Split a value into BF16 hi and lo (x = hi + lo to ~16 bits more).
 template <int PREC>
 __device__ __forceinline__ void split_to(const float (&v)[8], bf16x8& hi, bf16x8& lo) {
+  if constexpr (PREC == HPX_PREC_BF16X3) {
+    uint32_t h[4], l[4];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    if constexpr (PREC == HPX_PREC_BF16X3) {
-      __bf16 h, l;
-      split(v[q], h, l);
-      hi[q] = h;
-      lo[q] = l;
-    } else {
-      hi[q] = (__bf16)v[q];
-    }
+    for (int q = 0; q < 4; ++q) split2(v[2 * q], v[2 * q + 1], h[q], l[q]);
+    hi = __builtin_bit_cast(bf16x8, h);
+    lo = __builtin_bit_cast(bf16x8, l);
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) hi[q] = (__bf16)v[q];
   }
 }
 
@@ -580,7 +593,15 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
         if constexpr (PREC == HPX_PREC_FP32) {
           L.a[p][r][gk] = w[j];
         } else if constexpr (PREC == HPX_PREC_BF16X3) {
-          split(w[j], L.hi[p][r][bf16_slot(r, gk)], L.lo[p][r][bf16_slot(r, gk)]);
+          // pairs (j, j + 2): rows r and r + 1 of one parity
+          if ((j & 2) == 0) {
+            uint32_t h, l;
+            split2(w[j], w[j + 2], h, l);
+            L.hi[p][r][bf16_slot(r, gk)] = __builtin_bit_cast(bf16x2, h)[0];
+            L.lo[p][r][bf16_slot(r, gk)] = __builtin_bit_cast(bf16x2, l)[0];
+            L.hi[p][r + 1][bf16_slot(r + 1, gk)] = __builtin_bit_cast(bf16x2, h)[1];
+            L.lo[p][r + 1][bf16_slot(r + 1, gk)] = __builtin_bit_cast(bf16x2, l)[1];
+          }
         } else {
           L.hi[p][r][bf16_slot(r, gk)] = (__bf16)w[j];
         }
@@ -818,17 +839,8 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
           *reinterpret_cast<float4*>(&L.a[p][gk][c8 + 4]) = make_float4(w[p + 8], w[p + 10], w[p + 12], w[p + 14]);
         } else {
           bf16x8 hi, lo;
-#pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            if constexpr (PREC == HPX_PREC_BF16X3) {
-              __bf16 h, l;
-              split(w[p + 2 * q], h, l);
-              hi[q] = h;
-              lo[q] = l;
-            } else {
-              hi[q] = (__bf16)w[p + 2 * q];
-            }
-          }
+          const float wp[8] = {w[p], w[p + 2], w[p + 4], w[p + 6], w[p + 8], w[p + 10], w[p + 12], w[p + 14]};
+          split_to<PREC>(wp, hi, lo);
           const int slot = bf16_slot(gk, c8);
           *reinterpret_cast<bf16x8*>(&L.hi[p][gk][slot]) = hi;
           if constexpr (PREC == HPX_PREC_BF16X3) *reinterpret_cast<bf16x8*>(&L.lo[p][gk][slot]) = lo;
