@@ -567,23 +567,23 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
       v[2] = g.ct[kc + gk];
       return;
     }
-    const float* src = Psrc + kc;
+    // a row past the parity's count reads row 0 (valid memory) and is zeroed
+    // in store_a: the load itself is unconditional
+    const float* src = (srow_ok ? Psrc : Pm + sh) + kc;
     if (VEC && kc >= kpad) src += vstride - kpad;
-    if (srow_ok) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
-        v[4 * q] = t.x;
-        v[4 * q + 1] = t.y;
-        v[4 * q + 2] = t.z;
-        v[4 * q + 3] = t.w;
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = 0.f;
+    for (int q = 0; q < 4; ++q) {
+      const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
+      v[4 * q] = t.x;
+      v[4 * q + 1] = t.y;
+      v[4 * q + 2] = t.z;
+      v[4 * q + 3] = t.w;
     }
   };
-  auto store_a = [&](FwdLds<PREC>& L, const float* v) {
+  auto store_a = [&](FwdLds<PREC>& L, const float* va) {
+    float v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = (VEC && !srow_ok) ? 0.f : va[q];
     if constexpr (!VEC) {
       float w[kSegL];
       segment_recurrence(w, make_float2(v[0], v[1]), v[2], [&](int j) { return abl[gs][j]; });
@@ -633,8 +633,12 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
           bv[p][ct][q] = buffer_load(xr, voff,
                                      (int)((pb * pstride + (int64_t)(kk + frag_k_uniform<PREC>(q)) * nc + ct * 16) * 4));
     }
+  };
+  // the vector transform's lane signs, applied just before the products
+  // (writing a register that a load in flight targets would wait for it)
+  auto prep_b = [&](int kc, float (&bv)[2][CT][8]) {
     if constexpr (VEC) {
-      const float sg = half1 ? sgn1 : sgn0;
+      const float sg = kc >= kpad ? sgn1 : sgn0;
 #pragma unroll
       for (int p = 0; p < 2; ++p)
 #pragma unroll
@@ -669,6 +673,7 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
       const int c1 = min(c + 1, nchunk - 1);
       load_a(kof(c1), pa);
       load_b(kof(c1), b1);
+      prep_b(kof(c), b0);
       mfma_chunk<PREC, CT>(lds[0], b0, acc, lane);
       store_a(lds[1], pa);
       __syncthreads();
@@ -676,6 +681,7 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
       const int c2 = min(c + 2, nchunk - 1);
       load_a(kof(c2), pa);
       load_b(kof(c2), b0);
+      prep_b(kof(c + 1), b1);
       mfma_chunk<PREC, CT>(lds[1], b1, acc, lane);
       store_a(lds[0], pa);
       __syncthreads();
@@ -792,28 +798,27 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
   const int ckoff = gkq < kpad ? (gg * kpad + gkq) * 8 : nsg * kpad * 8;
 
   // table tile: this block's 64 ring pairs, i chunk starting at ic
+  bool a_ok = false;  // vector transform: the last loaded table piece lies in the table block
   auto load_a = [&](int ic, float* v) {
     if constexpr (!VEC) {  // the checkpoint; store_a generates
       v[0] = buffer_load(ckr, ckoff, (ic / 8) * kpad * 8);
       v[1] = buffer_load(ckr, ckoff + 4, (ic / 8) * kpad * 8);
       return;
     }
+    // an entry outside the table block reads the block's first (valid
+    // memory) and is zeroed in store_a: the load itself is unconditional
     const bool half1 = VEC && ic >= kp;
     const int ii = half1 ? ic - kp : ic;
     const int spad = half1 ? spad1 : spad0;
-    if (k0 + sr < kpad && ii + sh < spad) {
-      const float* src = (half1 ? Pbase1 : Pbase0) + (int64_t)k0 * spad + ii;
+    a_ok = k0 + sr < kpad && ii + sh < spad;
+    const float* src = a_ok ? (half1 ? Pbase1 : Pbase0) + (int64_t)k0 * spad + ii : PT + p_off[m];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
-        v[4 * q] = t.x;
-        v[4 * q + 1] = t.y;
-        v[4 * q + 2] = t.z;
-        v[4 * q + 3] = t.w;
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) v[q] = 0.f;
+    for (int q = 0; q < 4; ++q) {
+      const float4 t = *reinterpret_cast<const float4*>(src + 4 * q);
+      v[4 * q] = t.x;
+      v[4 * q + 1] = t.y;
+      v[4 * q + 2] = t.z;
+      v[4 * q + 3] = t.w;
     }
   };
   // the recurrence coefficients of the segment of chunk ic (wave-uniform,
@@ -848,15 +853,18 @@ __global__ __launch_bounds__(kThreads * RH, CT == 1 ? 4 : 2) void legendre_inv_k
       }
       return;
     }
+    float vz[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) vz[q] = a_ok ? v[q] : 0.f;
     if constexpr (PREC == HPX_PREC_FP32) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) L.a[sp][sr][sh + q] = v[q];
+      for (int q = 0; q < 16; ++q) L.a[sp][sr][sh + q] = vz[q];
     } else if constexpr (PREC == HPX_PREC_BF16X3) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) split(v[q], L.hi[sp][sr][shs + q], L.lo[sp][sr][shs + q]);
+      for (int q = 0; q < 16; ++q) split(vz[q], L.hi[sp][sr][shs + q], L.lo[sp][sr][shs + q]);
     } else {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) L.hi[sp][sr][shs + q] = (__bf16)v[q];
+      for (int q = 0; q < 16; ++q) L.hi[sp][sr][shs + q] = (__bf16)vz[q];
     }
   };
   // B fragments: rows i of aT (this m's rows of parity p).  Rows at or
