@@ -458,10 +458,13 @@ template <int WAVES, int TPF = 64>
 __device__ __forceinline__ void analysis_store(const StageArgs& a, const RingPair& pr, int k, int64_t b0,
                                                int64_t b, const float2* lds, int stride, const float2* phs,
                                                float* __restrict__ X, int lane) {
+  // phs[m] = exp(-i m phi0) c_m / 2, zero from K on.  With u = Z[m] and
+  // v = conj Z[N-m], the rings' spectra are Fn = phs (u + v) and
+  // Fs = phs (-i) (u - v), so S = Fn + Fs = phs (p - i q) and
+  // D = Fn - Fs = phs (p + i q) with p = u + v, q = u - v.
   constexpr int kThreadsW = WAVES * TPF;
   const int N = pr.nphi;
   const float2* buf = lds + (threadIdx.x / TPF) * stride;
-  const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
   if (a.layout == 0) {
     // folded spectra X[m][parity][k][2b + re/im]: fields fastest so that
@@ -472,35 +475,35 @@ __device__ __forceinline__ void analysis_store(const StageArgs& a, const RingPai
       const int m = e / WAVES;
       const int g = e - m * WAVES;
       if (b0 + g >= a.nbatch) continue;
-      {
-        float2 S = make_float2(0.f, 0.f), D = S;
-        if (m < K) {
-          const float2* seq = lds + g * stride;
-          const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
-          float2 Fn, Fs;
-          unpack_pair(seq[pad(m)], seq[pad(m == 0 ? 0 : N - m)], phs[m], c, Fn, Fs);
-          if (pr.single) {
-            S = Fn;
-            D = Fn;
-          } else {
-            S = cadd(Fn, Fs);
-            D = csub(Fn, Fs);
-          }
-        }
-        float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * (b0 + g);
-        *reinterpret_cast<float2*>(base) = S;
-        *reinterpret_cast<float2*>(base + pstride) = D;
+      const float2* seq = lds + g * stride;
+      const int mi = min(m, K - 1);  // past K the phase factor is zero
+      const float2 u = seq[pad(mi)];
+      const float2 zc = seq[pad(mi == 0 ? 0 : N - mi)];
+      const float2 ph = phs[m];
+      const float2 p = make_float2(u.x + zc.x, u.y - zc.y);
+      float2 S, D;
+      if (pr.single) {
+        S = cmul(p, ph);
+        D = S;
+      } else {
+        const float2 q = make_float2(u.x - zc.x, u.y + zc.y);
+        S = cmul(make_float2(p.x + q.y, p.y - q.x), ph);
+        D = cmul(make_float2(p.x - q.y, p.y + q.x), ph);
       }
+      float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * (b0 + g);
+      *reinterpret_cast<float2*>(base) = S;
+      *reinterpret_cast<float2*>(base + pstride) = D;
     }
   } else if (b < a.nbatch) {
     // ring-major ftm[b][t][m]: the field's threads along m
     float2* out = reinterpret_cast<float2*>(X) + b * a.nring * (int64_t)a.mmax;
     for (int m = lane; m < a.mmax; m += TPF) {
-      float2 Fn = make_float2(0.f, 0.f), Fs = Fn;
-      if (m < K) {
-        const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
-        unpack_pair(buf[pad(m)], buf[pad(m == 0 ? 0 : N - m)], phs[m], c, Fn, Fs);
-      }
+      const int mi = min(m, K - 1);
+      const float2 u = buf[pad(mi)];
+      const float2 zc = buf[pad(mi == 0 ? 0 : N - mi)];
+      const float2 ph = phs[m];
+      const float2 Fn = cmul(make_float2(u.x + zc.x, u.y - zc.y), ph);
+      const float2 Fs = cmul(make_float2(u.y + zc.y, zc.x - u.x), ph);
       out[(int64_t)k * a.mmax + m] = Fn;
       if (!pr.single) out[(int64_t)(a.nring - 1 - k) * a.mmax + m] = Fs;
     }
@@ -525,11 +528,18 @@ __global__ __launch_bounds__(WAVES * TPF) void ring_analysis_wave_kernel(const f
   const int64_t b = b0 + wave;
   const int stride = wave_stride(M);
   float2* buf = lds + wave * stride;
-  // the pair's phase factors exp(-i m phi0), staged once for the store loop
+  // the pair's phase factors exp(-i m phi0) times c_m / 2 (c_m: the
+  // quadrature weight and scale, doubled off DC and Nyquist in cmode 1;
+  // zero from K on), staged once for the store loop
   float2* phs = lds + WAVES * wave_stride(1 << HI);
   {
     const float2* ph = a.phase + (int64_t)k * a.mmax;
-    for (int m = threadIdx.x; m < a.mmax; m += kThreadsW) phs[m] = ph[m];
+    const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
+    const int K = min(a.mmax, N / 2 + 1);
+    for (int m = threadIdx.x; m < a.mmax; m += kThreadsW) {
+      const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
+      phs[m] = m < K ? cscale(ph[m], 0.5f * c) : make_float2(0.f, 0.f);
+    }
   }
   // with several wavefronts per field the DFT has work-group barriers, so
   // every field runs it (fields beyond the batch on zeros); the first pass
@@ -596,7 +606,10 @@ __global__ __launch_bounds__(WAVES * TPF) void ring_synthesis_wave_kernel(const 
   }
 
   const int stride = wave_stride(M);
-  for (int idx = threadIdx.x; idx < WAVES * stride; idx += kThreadsW) lds[idx] = make_float2(0.f, 0.f);
+  // the packing writes points m and N - m for m < K; when M == N and
+  // K = N/2 + 1 that is every point, and the sequences need no zero fill
+  if (M != N || 2 * (K - 1) < N)
+    for (int idx = threadIdx.x; idx < WAVES * stride; idx += kThreadsW) lds[idx] = make_float2(0.f, 0.f);
   __syncthreads();
   // the packing leaves the chirp factors to a per-wave pass below
   if (a.layout == 0) {
