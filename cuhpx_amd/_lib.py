@@ -98,6 +98,7 @@ SIGNATURES = {
     "hpx_sht_stage_ring_synthesis": (_c_int, [_vp, _vp, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_autotune": (_c_int, [_vp, _c_int, _c_i64, _vp, _c_i64, _vp]),
     "hpx_sht_tuned_shapes": (_c_int, [_vp, _c_int, _c_i64, _i32p]),
+    "hpx_sht_set_shapes": (_c_int, [_vp, _c_int, _c_i64, _i32p]),
     "hpx_sht_mixed_workspace_bytes": (_c_i64, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64)]),
     "hpx_sht_mixed_create": (_c_int, [_c_int, ctypes.POINTER(_vp), ctypes.POINTER(_c_i64), _c_int, _c_int,
                                       ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,

@@ -127,6 +127,16 @@ int autotune(ShtPlan& p, int precision, int64_t nbatch, float* ws, int64_t ws_by
       best_f = ms;
       gt.fwd_ct = ct;
     }
+    if (ct == kGemmCtChoices[1] && !p.vector && p.kpad % 64 == 0) {
+      // the 8-wave forward kernel (128 columns per work-group)
+      rc = time_ms([&] { return launch_gemm_fwd(p, precision, kGemmFwdW8, X, aT, nbatch, s); }, s, t.e0, t.e1,
+                   t.flush, &ms);
+      if (rc != HPX_OK) return rc;
+      if (ms < best_f) {
+        best_f = ms;
+        gt.fwd_ct = kGemmFwdW8;
+      }
+    }
     for (int rh : kGemmRhChoices) {
       if (p.vector && rh != 1) continue;  // vector GEMMs run 64 ring pairs per work-group
       rc = time_ms([&] { return launch_gemm_inv(p, precision, ct, rh, aT, X, nbatch, s); }, s, t.e0, t.e1, t.flush,
@@ -213,6 +223,42 @@ extern "C" int hpx_sht_tuned_shapes(const hpx_sht_plan* plan, int precision, int
   shapes[6] = g.fwd_ct;
   shapes[7] = g.inv_ct;
   shapes[8] = g.inv_rh;
+  return HPX_OK;
+}
+
+extern "C" int hpx_sht_set_shapes(hpx_sht_plan* plan, int precision, int64_t nbatch, const int32_t* shapes) {
+  ShtPlan* p = W(plan);
+  if (p == nullptr || shapes == nullptr) HPX_FAIL(HPX_ERR_ARG, "null argument");
+  if (precision < 0 || precision > 2) HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
+  if (nbatch < 1) HPX_FAIL(HPX_ERR_ARG, "nbatch must be >= 1");
+  FftTune f;
+  for (int d = 0; d < 2; ++d)
+    for (int c = 0; c < 3; ++c) {
+      const int w = shapes[3 * d + c], tpf = shapes[9 + 3 * d + c];
+      if (w != kFftWaveChoices[c][0] && w != kFftWaveChoices[c][1] && w != kFftWaveChoices[c][2])
+        HPX_FAIL(HPX_ERR_ARG, "no FFT kernel with %d fields per work-group for size class %d", w, c);
+      if ((tpf != 64 && tpf != 128 && tpf != 256) || w * tpf > 1024)
+        HPX_FAIL(HPX_ERR_ARG, "no FFT kernel with %d fields of %d threads", w, tpf);
+      if (fft_lds_bytes(*p, c, w) > 160 * 1024)
+        HPX_FAIL(HPX_ERR_UNSUPPORTED, "%d fields of size class %d exceed the LDS of a work-group", w, c);
+      f.waves[d][c] = w;
+      f.tpf[d][c] = tpf;
+    }
+  GemmTune g;
+  g.fwd_ct = shapes[6];
+  g.inv_ct = shapes[7];
+  g.inv_rh = shapes[8];
+  const bool fwd_ok = g.fwd_ct == kGemmCtChoices[0] || g.fwd_ct == kGemmCtChoices[1] ||
+                      (g.fwd_ct == kGemmFwdW8 && !p->vector && p->kpad % 64 == 0);
+  if (!fwd_ok) HPX_FAIL(HPX_ERR_ARG, "no forward GEMM with column tiles %d for this plan", g.fwd_ct);
+  if (g.inv_ct != kGemmCtChoices[0] && g.inv_ct != kGemmCtChoices[1])
+    HPX_FAIL(HPX_ERR_ARG, "no inverse GEMM with %d column tiles", g.inv_ct);
+  if (g.inv_rh != kGemmRhChoices[0] && g.inv_rh != kGemmRhChoices[1])
+    HPX_FAIL(HPX_ERR_ARG, "no inverse GEMM with %d row halves", g.inv_rh);
+  const int64_t ncol = col_pad(nbatch);
+  std::lock_guard<std::mutex> lk(p->tune_mu);
+  p->fft_tunes[ncol] = f;
+  p->gemm_tunes[{precision, ncol}] = g;
   return HPX_OK;
 }
 

@@ -698,6 +698,162 @@ __global__ __launch_bounds__(kThreads, gemm_min_waves(PREC, CT)) void legendre_f
                       });
 }
 
+// Forward GEMM with 8 wavefronts (scalar transform): the work-group covers
+// 128 columns (16 per wave) of 64 rows per parity, so one generated table
+// tile serves all of them, and a k chunk is 64 ring pairs (two 32-pair
+// halves, each a GemmLds tile), so every one of the 512 threads generates
+// one 16-degree segment for one ring pair per chunk: per ring pair and
+// column half the generation work and half the barriers of the 4-wave
+// kernel, at the same occupancy (2 work-groups of 64 KB per CU, 4 waves per
+// SIMD).  The spectra of a chunk's second half are loaded during the first
+// half's products, the next chunk's first half during the second's.  Needs
+// kpad % 64 == 0 (chunks start at multiples of 64, kmin rounded down).
+constexpr int kW8Threads = 512;
+template <int PREC>
+__global__ __launch_bounds__(kW8Threads, 4) void legendre_fwd_w8_kernel(
+    const float* __restrict__ X, float* __restrict__ aT, OtfArgs g, const int64_t* __restrict__ p_off,
+    const int* __restrict__ nl, const int* __restrict__ kmin, const int2* __restrict__ items, int nitems,
+    int ncb, int kpad, int64_t ncol) {
+  using Half = GemmLds<PREC, kRowsPerParity>;  // one 32-ring-pair half of a chunk
+  constexpr int kOutW = 128, kOutS = kOutW + 4;
+  constexpr size_t kTileBytes = sizeof(float) * 64 * kOutS;
+  constexpr size_t kLdsBytes = 4 * sizeof(Half) > kTileBytes ? 4 * sizeof(Half) : kTileBytes;
+  __shared__ __align__(16) unsigned char smem[kLdsBytes];
+  Half* lds = reinterpret_cast<Half*>(smem);  // lds[2 * buffer + half]
+  __shared__ float2 abl[8][kSegL];
+  const int idx = xcd_local_index(blockIdx.x, gridDim.x);
+  if (idx >= nitems * ncb) return;
+  const int item = idx / ncb;
+  const int cb = idx - item * ncb;
+  const int m = items[item].x;
+  const int jb = items[item].y;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  // a wave whose columns lie beyond the batch works on the first columns and
+  // stores nothing, so that every wave issues the same loads
+  const int col0 = cb * kOutW + wave * 16 < ncol ? cb * kOutW + wave * 16 : 0;
+  const int nlp[2] = {nl[2 * m], nl[2 * m + 1]};
+  const int padp0 = (nlp[0] + kRowPad - 1) / kRowPad * kRowPad;
+  const int row0 = jb * kRowsPerParity;
+  const int64_t pstride = (int64_t)kpad * ncol;
+  const int nc = (int)ncol;
+
+  floatx4 acc[2][4][1];
+#pragma unroll
+  for (int p = 0; p < 2; ++p)
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) acc[p][rt][0] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  // generation: thread (segment gs of this work-group's 8, ring pair gk of
+  // the chunk's 64; gh = gk / 32 the half it writes)
+  const int gs = threadIdx.x >> 6;
+  const int gk = threadIdx.x & 63;
+  const int gh = gk >> 5, gkk = gk & 31;
+  const int sbase = g.seg_off[m];
+  const int nsg = g.seg_off[m + 1] - sbase;
+  if (threadIdx.x < 8 * kSegL) {
+    const int ts = threadIdx.x / kSegL, tj = threadIdx.x % kSegL;
+    abl[ts][tj] = jb * 8 + ts < nsg ? g.ab[(int64_t)(sbase + jb * 8 + ts) * kSegL + tj] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  const __amdgpu_buffer_rsrc_t ckr =
+      buffer_rsrc(reinterpret_cast<const float*>(g.ck + (int64_t)sbase * kpad), (int64_t)nsg * kpad * 8);
+  const int ckoff = ((jb * 8 + gs) * kpad + gk) * 8;
+  const __amdgpu_buffer_rsrc_t xr = buffer_rsrc(X + (int64_t)m * 2 * pstride, 2 * pstride * 4);
+  const int bvoff = (frag_k_lane<PREC>(lane) * nc + col0 + (lane & 15)) * 4;
+
+  auto load_a = [&](int kc, float* v) {
+    v[0] = buffer_load(ckr, ckoff, kc * 8);
+    v[1] = buffer_load(ckr, ckoff + 4, kc * 8);
+    v[2] = g.ct[kc + gk];
+  };
+  auto store_a = [&](int buf, const float* v) {
+    Half& L = lds[2 * buf + gh];
+    float w[kSegL];
+    segment_recurrence(w, make_float2(v[0], v[1]), v[2], [&](int j) { return abl[gs][j]; });
+#pragma unroll
+    for (int j = 0; j < kSegL; ++j) {
+      const int p = j & 1, r = 8 * gs + (j >> 1);
+      if constexpr (PREC == HPX_PREC_FP32) {
+        L.a[p][r][gkk] = w[j];
+      } else if constexpr (PREC == HPX_PREC_BF16X3) {
+        if ((j & 2) == 0) {
+          uint32_t h, l;
+          split2(w[j], w[j + 2], h, l);
+          L.hi[p][r][bf16_slot(r, gkk)] = __builtin_bit_cast(bf16x2, h)[0];
+          L.lo[p][r][bf16_slot(r, gkk)] = __builtin_bit_cast(bf16x2, l)[0];
+          L.hi[p][r + 1][bf16_slot(r + 1, gkk)] = __builtin_bit_cast(bf16x2, h)[1];
+          L.lo[p][r + 1][bf16_slot(r + 1, gkk)] = __builtin_bit_cast(bf16x2, l)[1];
+        }
+      } else {
+        L.hi[p][r][bf16_slot(r, gkk)] = (__bf16)w[j];
+      }
+    }
+  };
+  auto load_b = [&](int kc, float (&bv)[2][1][8]) {
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        bv[p][0][q] = buffer_load(xr, bvoff, (int)((p * pstride + (int64_t)(kc + frag_k_uniform<PREC>(q)) * nc) * 4));
+  };
+
+  // chunks of 64 ring pairs from kmin[m] rounded down to 64.  The 32-pair
+  // halves below kmin[m] rounded down to 32 (what the ring analysis leaves
+  // unwritten for this m) are loaded but not multiplied, so the products
+  // run over the same 32-pair chunks in the same order as in the 4-wave
+  // kernel (bit-identical results).
+  const int k32 = min(kmin[m], kpad) / kKTile * kKTile;
+  const int c0 = k32 / 64;
+  const int nchunk = kpad / 64 - c0;
+  const bool skip_first_half = c0 * 64 < k32;
+  auto kof = [&](int t) { return (c0 + t) * 64; };
+  if (nchunk > 0) {
+    float pa[3];
+    float b0[2][1][8], b1[2][1][8];
+    load_a(kof(0), pa);
+    load_b(kof(0), b0);
+    store_a(0, pa);
+    __syncthreads();
+    int cur = 0;
+    for (int c = 0;; ++c) {
+      const int cn = min(c + 1, nchunk - 1);
+      load_a(kof(cn), pa);
+      load_b(kof(c) + 32, b1);
+      if (c > 0 || !skip_first_half) mfma_chunk<PREC, 1>(lds[2 * cur], b0, acc, lane);
+      load_b(kof(cn), b0);
+      mfma_chunk<PREC, 1>(lds[2 * cur + 1], b1, acc, lane);
+      store_a(cur ^ 1, pa);
+      __syncthreads();
+      if (c + 1 >= nchunk) break;
+      cur ^= 1;
+    }
+  }
+
+  // output: both parities' 64 x 128 tiles through LDS, whole row segments
+  float* T = reinterpret_cast<float*>(smem);
+  float* Ab = aT + (p_off[m] / kpad) * ncol + cb * kOutW;
+  const int64_t ncol_left = ncol - cb * kOutW;
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        T[(rt * 16 + 4 * (lane >> 4) + r) * kOutS + wave * 16 + (lane & 15)] = acc[p][rt][0][r];
+    __syncthreads();
+    for (int v = threadIdx.x; v < 64 * (kOutW / 4); v += kW8Threads) {
+      const int r = v / (kOutW / 4);
+      const int c4 = v - r * (kOutW / 4);
+      const int i = row0 + r;
+      if (i < nlp[p] && 4 * c4 < ncol_left)
+        *reinterpret_cast<float4*>(Ab + (int64_t)((p ? padp0 : 0) + i) * ncol + 4 * c4) =
+            *reinterpret_cast<const float4*>(&T[r * kOutS + 4 * c4]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Inverse: Y[m][p][k][col] = sum_i P[m][p][i][k] aT[m][p][i][col]
 // ---------------------------------------------------------------------------
@@ -1142,6 +1298,26 @@ int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, flo
                     int64_t nbatch, hipStream_t s) {
   if (nbatch <= 0 || p.n_fwd_items == 0) return HPX_OK;
   const int64_t ncol = col_pad(nbatch);
+  if (ct == kGemmFwdW8) {
+    if (p.vector || p.kpad % 64 != 0)
+      HPX_FAIL(HPX_ERR_UNSUPPORTED, "the 8-wave forward GEMM needs a scalar plan with ring pairs padded to 64");
+    const int64_t ncb8 = (ncol + 127) / 128;
+    if ((int64_t)p.n_fwd_items * ncb8 > 0x7ffffff0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+    const dim3 grid8(xcd_grid((int64_t)p.n_fwd_items * ncb8));
+    const OtfArgs otf8{p.d_ck, p.d_ab, p.d_seg_off, p.d_ct};
+#define HPX_FWD8(PR)                                                                                         \
+  legendre_fwd_w8_kernel<PR><<<grid8, kW8Threads, 0, s>>>(X, aT, otf8, p.d_p_off, p.d_nl, p.d_kmin, p.d_fwd_items, \
+                                                         p.n_fwd_items, (int)ncb8, p.kpad, ncol)
+    switch (precision) {
+      case HPX_PREC_FP32: HPX_FWD8(HPX_PREC_FP32); break;
+      case HPX_PREC_BF16X3: HPX_FWD8(HPX_PREC_BF16X3); break;
+      case HPX_PREC_BF16: HPX_FWD8(HPX_PREC_BF16); break;
+      default: HPX_FAIL(HPX_ERR_ARG, "unknown precision %d", precision);
+    }
+#undef HPX_FWD8
+    HPX_LAUNCH_CHECK();
+    return HPX_OK;
+  }
   const int64_t ncb = (ncol + 64 * ct - 1) / (64 * ct);
   if ((int64_t)p.n_fwd_items * ncb > 0x7ffffff0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
   const dim3 grid(xcd_grid((int64_t)p.n_fwd_items * ncb));

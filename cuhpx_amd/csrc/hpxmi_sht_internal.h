@@ -32,6 +32,9 @@ constexpr int kFftWaveChoices[3][3] = {{4, 8, 16}, {2, 4, 8}, {1, 2, 4}};
 // each pass (a work-group barrier between passes); fields * threads <= 1024.
 constexpr int kFftTpfChoices[3] = {64, 128, 256};
 constexpr int kGemmCtChoices[2] = {1, 2};
+// Forward GEMM "column tiles" value that selects the 8-wave kernel (128
+// columns per work-group, 64-ring-pair chunks; scalar plans with kpad % 64 == 0).
+constexpr int kGemmFwdW8 = 4;
 constexpr int kGemmRhChoices[2] = {1, 2};
 
 // One mirror pair of rings (north t, south nring-1-t); the equator of a grid

@@ -197,6 +197,16 @@ class ShtPlan:
                 "gemm_inv_col_tiles": v[7], "gemm_inv_row_blocks": v[8],
                 "fft_analysis_threads_per_field": v[9:12], "fft_synthesis_threads_per_field": v[12:15]}
 
+    def set_shapes(self, precision, nbatch, shapes):
+        """Set the launch shapes used for (precision, nbatch) without timing
+        (a dict in the form tuned_shapes returns; values are validated)."""
+        v = (list(shapes["fft_analysis_waves"]) + list(shapes["fft_synthesis_waves"])
+             + [shapes["gemm_fwd_col_tiles"], shapes["gemm_inv_col_tiles"], shapes["gemm_inv_row_blocks"]]
+             + list(shapes["fft_analysis_threads_per_field"]) + list(shapes["fft_synthesis_threads_per_field"]))
+        arr = (ctypes.c_int32 * 15)(*v)
+        rc = _lib.lib().hpx_sht_set_shapes(self._handle, _lib.PRECISIONS[precision], nbatch, arr)
+        _lib.check(rc, "SHT set shapes")
+
     def _run(self, fn, *args, nbatch, f64=False):
         ws, nbytes = self.workspace(nbatch, f64)
         with torch.cuda.device(self.device):

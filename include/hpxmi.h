@@ -353,11 +353,19 @@ int hpx_sht_autotune(hpx_sht_plan* plan, int precision, int64_t nbatch, void* wo
 
 /* The launch shapes used for (precision, nbatch), 15 values: shapes[0..2]
  * analysis fields per work-group of FFT classes 0..2, shapes[3..5] the same
- * for synthesis, shapes[6] / shapes[7] forward / inverse GEMM column tiles,
+ * for synthesis, shapes[6] / shapes[7] forward / inverse GEMM column tiles
+ * (forward: 4 = the 8-wavefront kernel, 128 columns per work-group),
  * shapes[8] inverse GEMM ring-pair rows per work-group in units of 64,
  * shapes[9..14] threads per field (64, 128 or 256) of the six FFT entries 0..5. */
 int hpx_sht_tuned_shapes(const hpx_sht_plan* plan, int precision, int64_t nbatch,
                          int32_t* shapes);
+
+/* Sets the launch shapes used for (precision, nbatch) in the layout of
+ * hpx_sht_tuned_shapes, without timing anything (a saved tuning result, or
+ * one kernel variant chosen on purpose); every value is checked against the
+ * kernels that exist and the plan's LDS.  Launch shapes never change
+ * results.  No counterpart in the reference. */
+int hpx_sht_set_shapes(hpx_sht_plan* plan, int precision, int64_t nbatch, const int32_t* shapes);
 
 /* A mixed-size request: nplans HEALPix (or ring-grid) plans, each with its
  * own batch of maps, run as one HIP graph whose branches (one per plan) run
