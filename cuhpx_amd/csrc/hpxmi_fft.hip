@@ -596,14 +596,16 @@ __global__ __launch_bounds__(WAVES * TPF) void ring_synthesis_wave_kernel(const 
       }
     }
   };
-  if (a.layout == 0) load_batch(threadIdx.x);
-
-  // the pair's phase factors exp(-i m phi0), staged once for the packing
+  // the pair's phase factors exp(-i m phi0), staged once for the packing.
+  // They are requested before the spectra: storing them to LDS waits for
+  // their loads, which would otherwise also wait for every spectra load
+  // issued before them (ring synthesis at nside 256, 64 fields: 184 -> 179 us)
   float2* phs = lds + WAVES * wave_stride(1 << HI);
   {
     const float2* ph = a.phase + (int64_t)k * a.mmax;
     for (int m = threadIdx.x; m < a.mmax; m += kThreadsW) phs[m] = ph[m];
   }
+  if (a.layout == 0) load_batch(threadIdx.x);
 
   const int stride = wave_stride(M);
   // the packing writes points m and N - m for m < K; when M == N and
