@@ -259,6 +259,106 @@ __global__ __launch_bounds__(TBMAX * TBMAX / PT) void remap_tiles(
   }
 }
 
+// The same tiles with loading and storing split between the halves of the
+// work-group: waves 0-7 load a row and store it into LDS, waves 8-15 read
+// the previous row from LDS and store it to memory, one row per step, with
+// one barrier per step (double-buffered LDS).  vmcnt counts a wave's loads
+// and stores in issue order, so a wave that both loads and stores waits, to
+// place a row's pixels in LDS, for its previous row's stores too; here the
+// loading waves issue no stores and the storing waves never wait on memory.
+// Whole tiles only (tb = TB): every thread moves kPer pixels per row.
+template <typename T, int TB, bool NTL, bool NTS>
+__global__ __launch_bounds__(1024) void remap_tiles_split(const T* __restrict__ in, T* __restrict__ out,
+                                                        int64_t nside, int order, int tiles_per_face,
+                                                        int64_t nrows, int rows_per_chunk, Side src, Side dst) {
+  constexpr int kHalf = 512;
+  constexpr int kPer = TB * TB / kHalf;
+  __shared__ T lds[2][TB * TB];
+  const int tile = xcd_local_index(blockIdx.x, gridDim.x);
+  TileGeom g;
+  g.face = tile / tiles_per_face;
+  const int b = tile - g.face * tiles_per_face;
+  int log_tb = 0;
+  while ((1 << log_tb) < TB) ++log_tb;
+  g.tb = TB;
+  g.log_tb = log_tb;
+  g.x0 = compress_bits(b) << log_tb;
+  g.y0 = compress_bits(b >> 1) << log_tb;
+  g.nest_base = ((int64_t)g.face << (2 * order)) + ((int64_t)b << (2 * log_tb));
+  const int mask = (int)nside - 1;
+  // wave-uniform role
+  const bool loader = __builtin_amdgcn_readfirstlane(threadIdx.x) < kHalf;
+  const Side side = loader ? src : dst;
+  int fx0 = 0, fy0 = 0;
+  if (side.scheme == kFlat) {
+    int ax = g.x0, ay = g.y0, bx = g.x0 + TB - 1, by = g.y0 + TB - 1;
+    correct(ax, ay, side.flip, side.k, mask);
+    correct(bx, by, side.flip, side.k, mask);
+    fx0 = min(ax, bx);
+    fy0 = min(ay, by);
+  }
+  const int t = threadIdx.x & (kHalf - 1);
+  // LDS slots (< 2^14) two to a register
+  int gidx[kPer];
+  uint32_t slot2[kPer / 2] = {};
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    int lx, ly;
+    int64_t gi;
+    element(side, g, t + kHalf * i, nside, order, fx0, fy0, lx, ly, gi);
+    slot2[i / 2] |= (uint32_t)(spread_bits(lx) | (spread_bits(ly) << 1)) << (16 * (i & 1));
+    gidx[i] = (int)gi;
+  }
+  auto slot = [&](int i) { return (slot2[i / 2] >> (16 * (i & 1))) & 0xffff; };
+  const int64_t npix = 12 * nside * nside;
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+  int64_t r1 = r0 + rows_per_chunk;
+  if (r1 > nrows) r1 = nrows;
+  const int nsteps = (int)(r1 - r0) + 1;
+  for (int s = 0; s < nsteps; ++s) {
+    if (loader) {
+      if (s + 1 < nsteps) {
+        const T* row = in + (r0 + s) * npix;
+        T v[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) v[i] = NTL ? __builtin_nontemporal_load(row + gidx[i]) : row[gidx[i]];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) lds[s & 1][slot(i)] = v[i];
+      }
+    } else if (s > 0) {
+      T* row = out + (r0 + s - 1) * npix;
+#pragma unroll
+      for (int i = 0; i < kPer; ++i) {
+        const T v = lds[(s - 1) & 1][slot(i)];
+        if constexpr (NTS)
+          __builtin_nontemporal_store(v, row + gidx[i]);
+        else
+          row[gidx[i]] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, int TB, bool NTL, bool NTS>
+int launch_split(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Side dst,
+                 hipStream_t stream, int rows_per_wg) {
+  int order = 0;
+  while ((int64_t(1) << order) < nside) ++order;
+  const int tiles_per_face = (int)((nside / TB) * (nside / TB));
+  const int ntiles = 12 * tiles_per_face;
+  int64_t chunks = (nrows + rows_per_wg - 1) / rows_per_wg;
+  if (chunks > 65535) chunks = 65535;  // grid y limit
+  const int rows_per_chunk = (int)((nrows + chunks - 1) / chunks);
+  chunks = (nrows + rows_per_chunk - 1) / rows_per_chunk;
+  dim3 grid(ntiles, (unsigned)chunks);
+  remap_tiles_split<T, TB, NTL, NTS><<<grid, 1024, 0, stream>>>(static_cast<const T*>(in), static_cast<T*>(out),
+                                                                nside, order, tiles_per_face, nrows, rows_per_chunk,
+                                                                src, dst);
+  HPX_LAUNCH_CHECK();
+  return HPX_OK;
+}
+
 // NEST -> RING through tiles that follow the rings.  A square tile's ring
 // segments average TB/2 pixels, and a RING destination written in such short
 // runs is slow (nest2ring 3.3 TB/s at nside 1024, against 5.2 TB/s for the
@@ -482,6 +582,12 @@ int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Si
   // the tile's index arithmetic too often (nside 512, 64 rows: 2.9 TB/s
   // against 5.6 with 4 rows; 2 rows 4.4-4.6, 8 rows 5.4-5.5).  The RING
   // kinds gained nothing from either change.
+  if constexpr (sizeof(T) == 4) {
+    if (!ring && nside >= 64) {
+      if (nest_flat) return launch_split<T, 64, true, true>(in, out, nside, nrows, src, dst, stream, 4);
+      return launch_split<T, 64, false, false>(in, out, nside, nrows, src, dst, stream, 4);
+    }
+  }
   if (!ring && sizeof(T) <= 4) {
     if (nest_flat) return launch_tiles<T, 64, 4, true, true>(in, out, nside, nrows, src, dst, stream, 0, 4);
     return launch_tiles<T, 64, 4, false, false>(in, out, nside, nrows, src, dst, stream, 0, 4);

@@ -159,6 +159,31 @@ def test_nest2ring_band_tiles(nside, rows, dtype):
     np.testing.assert_array_equal(got, _expected("nest2ring", x.numpy(), nside))
 
 
+@pytest.mark.parametrize("nside,rows", [(64, 1), (64, 7), (128, 5), (256, 6)])
+def test_nest_flat_4byte_ragged_rows(nside, rows):
+    """NEST <-> flat and flat -> flat with 4-byte pixels from nside 64 run
+    64 x 64 tiles whose loads and stores are done by separate halves of the
+    work-group, four rows per work-group: row counts that leave a short last
+    chunk, all four origins both ways, and float32 bit patterns, against the
+    oracle."""
+    x = _payload(nside, rows).to(torch.int32)
+    xd = x.cuda()
+    for o in ORIGINS:
+        for cw in (False, True):
+            print(f"[4-byte tiles] nside={nside} rows={rows} origin={o} cw={cw}", flush=True)
+            for kind in ("nest2flat", "flat2nest"):
+                got = _call(kind, xd, nside, o, cw).cpu().numpy()
+                np.testing.assert_array_equal(got, _expected(kind, x.numpy(), nside, o, cw),
+                                              err_msg=f"{kind} nside={nside} rows={rows} {o}{cw}")
+            o2, cw2 = ORIGINS[(ORIGINS.index(o) + 1) % 4], not cw
+            got = _call("flat2flat", xd, nside, o, cw, o2, cw2).cpu().numpy()
+            np.testing.assert_array_equal(got, _expected("flat2flat", x.numpy(), nside, o, cw, o2, cw2))
+    xf = torch.randn(rows, 12 * nside * nside, device="cuda")
+    got = cuhpx_amd.nest2flat(xf, "N", True, nside)
+    exp = _expected("nest2flat", xf.view(torch.int32).cpu().numpy(), nside, "N", True)
+    np.testing.assert_array_equal(got.view(torch.int32).cpu().numpy(), exp)
+
+
 @pytest.mark.parametrize("nside", [512, 1024])
 def test_flat2ring_large_nside_all_origins(nside):
     """flat2ring (a rotated / mirrored flat source, a RING destination) at the
