@@ -382,6 +382,33 @@ int launch_split(const void* in, void* out, int64_t nside, int64_t nrows, Side s
 // 256 x 64 tiles take 0.40 ms against 0.46 ms.
 constexpr int kBandThreads = 1024;
 
+// Raw buffer load of one 1-, 2- or 4-byte pixel (gfx9 descriptor, bounds
+// checked against the descriptor's byte count): the address is a base in
+// scalar registers, a wave-uniform scalar offset and one per-lane offset.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pixel_rsrc(const void* base, int nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, nbytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T pixel_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else if constexpr (sizeof(T) == 2)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b8(r, voff, soff, 0));
+}
+
+// Non-temporal raw buffer store of one pixel (cache policy 2: nt).
+template <typename T>
+__device__ __forceinline__ void pixel_store_nt(T v, __amdgpu_buffer_rsrc_t r, int voff) {
+  if constexpr (sizeof(T) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, voff, 0, 2);
+  else if constexpr (sizeof(T) == 2)
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(uint16_t, v), r, voff, 0, 2);
+  else
+    __builtin_amdgcn_raw_buffer_store_b8(__builtin_bit_cast(uint8_t, v), r, voff, 0, 2);
+}
+
 __device__ __forceinline__ uint32_t band_slot(const uint32_t* q, int i) {
   return (q[i / 2] >> (16 * (i & 1))) & 0xffff;
 }
@@ -469,6 +496,148 @@ __global__ __launch_bounds__(kBandThreads) void nest2ring_bands(
   }
 }
 
+// nest2ring_bands for 1- and 2-byte pixels.  There the kernel above
+// computes a 64-bit address per pixel and spills them (its pixel loads and
+// stores then wait on scratch reloads); this one addresses the NEST source
+// through a per-face buffer descriptor (the block base wave-uniform in a
+// scalar register, the lane's pixel as the vector offset) and the RING
+// destination through 32-bit offsets from the band's lowest RING index, and
+// requests row r + 1 before row r is written out.  nside 1024, [64, npix]:
+// int16 2.37 -> 1.04 ms, uint8 2.02 -> 0.96 ms (bit-exact).  For 4-byte
+// pixels it was slower on one box and faster on another (float32 1.48 ->
+// 1.60 / 1.48 -> 1.42 ms), and slower with few rows per chunk (1024 x 7:
+// 0.23 -> 0.27 ms), so 4-byte pixels keep the kernel above.
+template <typename T, int BW, int BS>
+__global__ __launch_bounds__(kBandThreads) void nest2ring_bands_narrow(
+    const T* __restrict__ in, T* __restrict__ out, int64_t nside, int xblocks, int sblocks,
+    int64_t nrows, int rows_per_chunk) {
+  constexpr int kPitch = BW + 8;
+  constexpr int kBlocksPerCol = BS / 8 + 1;
+  constexpr int kRingPer = BS * BW / kBandThreads;                   // RING pixels per thread
+  constexpr int kNestPer = (BW / 8) * kBlocksPerCol * 64 / kBandThreads;  // NEST slots per thread
+  static_assert(kRingPer % 2 == 0 && kNestPer % 2 == 0 && BS * kPitch < 0xffff, "band shape");
+  __shared__ T lds[2][BS * kPitch];
+  const int n = (int)nside;
+  const int tile = xcd_local_index(blockIdx.x, gridDim.x);
+  const int per_face = xblocks * sblocks;
+  const int face = tile / per_face;
+  const int rem = tile - face * per_face;
+  const int xb = rem / sblocks;
+  const int x0 = xb * BW;
+  const int s0 = x0 + (rem - xb * sblocks) * BS;
+
+  // Source: NEST, whole 8 x 8 Morton blocks.  Block (bx, by) covers
+  // s in [8 (bx + by), 8 (bx + by) + 14]; it meets the band for
+  // bx + by in [s0 / 8 - 1, s0 / 8 + BS / 8 - 1].  Slot e of the source
+  // side is pixel l = e % 64 of block e / 64, so a wavefront reads one
+  // whole block: the block's base index is wave-uniform (kept in scalar
+  // registers) and a lane adds its own l.  Lanes whose pixel lies outside
+  // the band still load it (every block inside the face is in the map; a
+  // block outside the face reads the row's first 64 pixels), and skip the
+  // LDS store: loads under a branch would make the compiler wait for every
+  // load and store in flight, the previous row's stores included.
+  // LDS slots (< BS * kPitch) are kept two to a register, 0xffff marking a
+  // slot outside the band: 1024 threads leave 128 VGPRs each.
+  uint32_t src_q[kNestPer / 2] = {}, dst_q[kRingPer / 2] = {};
+  int src_off[kNestPer];  // byte offset of the block in its face (< 2^31)
+  const int face_base = face * n * n;  // < 2^31 for nside <= 8192
+  const int l = threadIdx.x & 63;
+  const int lx = compress_bits(l), ly = compress_bits(l >> 1);
+#pragma unroll
+  for (int i = 0; i < kNestPer; ++i) {
+    const int blk = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6) + i * (kBandThreads / 64));
+    const int col = blk / kBlocksPerCol;
+    const int bx = (x0 >> 3) + col;
+    const int by = (s0 >> 3) - 1 - bx + (blk - col * kBlocksPerCol);
+    const bool in_face = by >= 0 && by < (n >> 3);
+    const int ix = 8 * bx + lx;
+    const int t = ix + 8 * by + ly - s0;
+    const bool ok = in_face && t >= 0 && t < BS;
+    src_q[i / 2] |= (uint32_t)(ok ? t * kPitch + (ix - x0) : 0xffff) << (16 * (i & 1));
+    src_off[i] = in_face ? ((spread_bits(bx) | (spread_bits(by) << 1)) << 6) * (int)sizeof(T) : 0;
+  }
+  // Destination: RING, ring t of the band, pixel p along it (increasing ix).
+  // The band's pixels lie in at most BS consecutive rings, so they are
+  // addressed as byte offsets from the band's lowest RING index (band_lo,
+  // found once per work-group): 32-bit buffer offsets instead of a 64-bit
+  // address per pixel.
+  __shared__ int band_lo;
+  if (threadIdx.x == 0) band_lo = 0x7fffffff;
+  int dst_i[kRingPer];
+  int lo = 0x7fffffff;
+#pragma unroll
+  for (int i = 0; i < kRingPer; ++i) {
+    const int e = threadIdx.x + i * kBandThreads;
+    const int t = e / BW;
+    const int p = e - t * BW;
+    const int ix = x0 + p;
+    const int iy = s0 + t - ix;
+    const bool ok = iy >= 0 && iy < n;
+    dst_q[i / 2] |= (uint32_t)(ok ? t * kPitch + p : 0xffff) << (16 * (i & 1));
+    dst_i[i] = ok ? (int)xyf2ring(ix, iy, face, nside) : 0;
+    if (ok) lo = min(lo, dst_i[i]);
+  }
+  __syncthreads();
+  atomicMin(&band_lo, lo);
+  __syncthreads();
+  lo = __builtin_amdgcn_readfirstlane(band_lo);
+#pragma unroll
+  for (int i = 0; i < kRingPer; ++i)
+    dst_i[i] = band_slot(dst_q, i) != 0xffff ? (dst_i[i] - lo) * (int)sizeof(T) : 0;
+
+  const int64_t npix = 12 * nside * nside;
+  const int64_t r0 = (int64_t)blockIdx.y * rows_per_chunk;
+  int64_t r1 = r0 + rows_per_chunk;
+  if (r1 > nrows) r1 = nrows;
+  // two register sets: row r + 1 is requested before row r is written out,
+  // so its loads are in flight while row r passes through LDS and leaves
+  // (the reload past the last row reads the last row again, unused)
+  T va[kNestPer], vb[kNestPer];
+  const int voff = l * (int)sizeof(T);
+  const int face_bytes = n * n * (int)sizeof(T);  // < 2^31 for nside <= 8192
+  auto load_row = [&](T (&v)[kNestPer], int64_t r) {
+    const __amdgpu_buffer_rsrc_t face_rsrc = pixel_rsrc(in + (r < r1 ? r : r1 - 1) * npix + face_base, face_bytes);
+#pragma unroll
+    for (int i = 0; i < kNestPer; ++i) v[i] = pixel_load<T>(face_rsrc, voff, src_off[i]);
+  };
+  auto put_row = [&](const T (&v)[kNestPer], int b) {
+#pragma unroll
+    for (int i = 0; i < kNestPer; ++i) {
+      const uint32_t q = band_slot(src_q, i);
+      if (q != 0xffff) lds[b][q] = v[i];
+    }
+  };
+  // the band's bytes run from RING index lo to at most BS + 1 rings of
+  // 4 nside pixels on (a few MB); the descriptor allows all of that
+  const int band_bytes = (int)min<int64_t>((npix - lo) * (int64_t)sizeof(T), 0x7fffffff);
+  auto emit_row = [&](int b, int64_t r) {
+    const __amdgpu_buffer_rsrc_t band_rsrc = pixel_rsrc(out + r * npix + lo, band_bytes);
+#pragma unroll
+    for (int i = 0; i < kRingPer; ++i) {
+      const uint32_t q = band_slot(dst_q, i);
+      if (q != 0xffff) pixel_store_nt<T>(lds[b][q], band_rsrc, dst_i[i]);
+    }
+  };
+  if (r0 >= r1) return;
+  load_row(va, r0);
+  int64_t r = r0;
+  for (; r + 1 < r1; r += 2) {
+    load_row(vb, r + 1);
+    put_row(va, 0);
+    __syncthreads();
+    emit_row(0, r);
+    load_row(va, r + 2);
+    put_row(vb, 1);
+    __syncthreads();
+    emit_row(1, r + 1);
+  }
+  if (r < r1) {
+    put_row(va, 0);
+    __syncthreads();
+    emit_row(0, r);
+  }
+}
+
 template <typename T, int BW, int BS>
 int launch_bands(const void* in, void* out, int64_t nside, int64_t nrows, hipStream_t stream) {
   if (nside < BW) HPX_FAIL(HPX_ERR_UNSUPPORTED, "band tiles need nside >= %d", BW);
@@ -485,9 +654,13 @@ int launch_bands(const void* in, void* out, int64_t nside, int64_t nrows, hipStr
   const int rows_per_chunk = (int)((nrows + chunks - 1) / chunks);
   chunks = (nrows + rows_per_chunk - 1) / rows_per_chunk;
   dim3 grid(ntiles, (unsigned)chunks);
-  nest2ring_bands<T, BW, BS><<<grid, kBandThreads, 0, stream>>>(
-      static_cast<const T*>(in), static_cast<T*>(out), nside, xblocks, sblocks, nrows,
-      rows_per_chunk);
+  if constexpr (sizeof(T) < 4)
+    nest2ring_bands_narrow<T, BW, BS><<<grid, kBandThreads, 0, stream>>>(
+        static_cast<const T*>(in), static_cast<T*>(out), nside, xblocks, sblocks, nrows, rows_per_chunk);
+  else
+    nest2ring_bands<T, BW, BS><<<grid, kBandThreads, 0, stream>>>(
+        static_cast<const T*>(in), static_cast<T*>(out), nside, xblocks, sblocks, nrows,
+        rows_per_chunk);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
