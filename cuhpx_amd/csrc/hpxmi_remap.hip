@@ -733,34 +733,40 @@ int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Si
       if (nside >= 1024) return launch_bands<T, 512, 32>(in, out, nside, nrows, stream);
       if (nside == 512) return launch_bands<T, 256, 64>(in, out, nside, nrows, stream);
     }
-    if (ring && nside >= 128) {
-      // RING -> NEST / flat: the destination streams whole lines, stored
-      // non-temporally (ring2nest 1.33 -> 1.25 ms at nside 1024, 64 rows);
-      // non-temporal loads of a NEST / flat source measured slower.  Unlike
-      // the NEST / flat kinds these gain from long row loops: at least 16
-      // rows per work-group (64 rows of [rows, npix] float32, ring2nest /
-      // nest2ring at nside 256: 3.6 / 3.0 TB/s with 4 rows, 4.5 / 3.6 with
-      // 16, 3.7 / 2.8 with 32; nside 128: 1.5 -> 1.7 TB/s; nside >= 512
-      // already ran 16-64 rows per work-group)
-      if (src.scheme == kRing)
-        return launch_tiles<T, 128, 16, false, true>(in, out, nside, nrows, src, dst, stream, 768, 0, 16);
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (ring && dst.scheme == kRing && nside >= 512) {
+      // flat -> RING with 4-byte pixels from nside 512: 128 x 128 tiles, whose
+      // RING runs average 64 pixels, with both roles in every wave and at
+      // least 16 rows per work-group (flat2ring at nside 1024, [64, npix]
+      // float32: 1.95 ms, against 2.20 ms with the split 64 x 64 tiles below)
       return launch_tiles<T, 128, 16, false, false>(in, out, nside, nrows, src, dst, stream, 768, 0, 16);
     }
   }
-  // NEST <-> flat and flat -> flat: 64 x 64 tiles, 4 pixels per thread (1024
-  // threads) and 4 rows per work-group: more, shorter-lived work-groups keep
-  // more loads in flight (nside 1024, [64, npix] float32: nest2flat
-  // 5.2 -> 5.8 TB/s, flat2flat 4.9 -> 5.4 TB/s, against 2048 work-groups of
-  // 256 threads looping over all 64 rows).  One row per work-group repeats
-  // the tile's index arithmetic too often (nside 512, 64 rows: 2.9 TB/s
-  // against 5.6 with 4 rows; 2 rows 4.4-4.6, 8 rows 5.4-5.5).  The RING
-  // kinds gained nothing from either change.
-  if constexpr (sizeof(T) == 4) {
-    if (!ring && nside >= 64) {
-      if (nest_flat) return launch_split<T, 64, true, true>(in, out, nside, nrows, src, dst, stream, 4);
-      return launch_split<T, 64, false, false>(in, out, nside, nrows, src, dst, stream, 4);
-    }
+  // Every other kind from nside 64 (RING kinds from 128): 64 x 64 tiles whose
+  // loads and stores are done by separate wave halves (remap_tiles_split),
+  // 4 rows per work-group for the NEST / flat kinds and 8 for the RING kinds,
+  // non-temporal loads and stores between NEST and flat and non-temporal
+  // stores of a NEST / flat destination written from RING.  Measured at
+  // nside 1024, [64, npix], against 64 x 64 tiles with both roles in every
+  // wave (NEST / flat kinds) or 128 x 128 tiles (RING kinds):
+  //   float32 nest2flat 1.20 -> 1.11 ms, flat2flat 1.26 -> 1.18, ring2nest
+  //   1.25 -> 1.25, ring2flat 1.32 -> 1.29 (nside 256: ring2nest 0.090 ->
+  //   0.074, ring2flat 0.114 -> 0.074 ms);
+  //   int16 nest2flat 1.47 -> 0.82, ring2nest 1.87 -> 0.84, flat2ring
+  //   2.04 -> 1.01; uint8 1.40 -> 0.73, 1.54 -> 0.69; int64 4.28 -> 2.28,
+  //   ring2nest 4.71 -> 2.40, nest2ring 5.04 -> 3.69 ms.
+  // (The earlier shapes: 4 rows per work-group beat 64 rows per work-group
+  // for the NEST / flat kinds, nside 512: 2.9 TB/s with one row, 5.6 with 4.)
+  if (ring && nside >= 128) {
+    if (src.scheme == kRing) return launch_split<T, 64, false, true>(in, out, nside, nrows, src, dst, stream, 8);
+    return launch_split<T, 64, false, false>(in, out, nside, nrows, src, dst, stream, 8);
   }
+  if (!ring && nside >= 64) {
+    if (nest_flat) return launch_split<T, 64, true, true>(in, out, nside, nrows, src, dst, stream, 4);
+    return launch_split<T, 64, false, false>(in, out, nside, nrows, src, dst, stream, 4);
+  }
+  // small maps: tiles of the whole face side with both roles in every wave
   if (!ring && sizeof(T) <= 4) {
     if (nest_flat) return launch_tiles<T, 64, 4, true, true>(in, out, nside, nrows, src, dst, stream, 0, 4);
     return launch_tiles<T, 64, 4, false, false>(in, out, nside, nrows, src, dst, stream, 0, 4);

@@ -159,18 +159,20 @@ def test_nest2ring_band_tiles(nside, rows, dtype):
     np.testing.assert_array_equal(got, _expected("nest2ring", x.numpy(), nside))
 
 
-@pytest.mark.parametrize("nside,rows", [(64, 1), (64, 7), (128, 5), (256, 6)])
-def test_nest_flat_4byte_ragged_rows(nside, rows):
-    """NEST <-> flat and flat -> flat with 4-byte pixels from nside 64 run
-    64 x 64 tiles whose loads and stores are done by separate halves of the
-    work-group, four rows per work-group: row counts that leave a short last
-    chunk, all four origins both ways, and float32 bit patterns, against the
-    oracle."""
-    x = _payload(nside, rows).to(torch.int32)
+@pytest.mark.parametrize("nside,rows,dtype", [(64, 1, torch.int32), (64, 7, torch.int32), (128, 5, torch.int32),
+                                              (256, 6, torch.int32), (64, 3, torch.uint8), (128, 5, torch.int16),
+                                              (128, 5, torch.int64)])
+def test_nest_flat_split_tiles_ragged_rows(nside, rows, dtype):
+    """NEST <-> flat and flat -> flat from nside 64 run 64 x 64 tiles whose
+    loads and stores are done by separate halves of the work-group, four rows
+    per work-group: row counts that leave a short last chunk, 1-, 2-, 4- and
+    8-byte pixels, all four origins both ways, and float32 bit patterns,
+    against the oracle."""
+    x = _payload(nside, rows).to(dtype)
     xd = x.cuda()
     for o in ORIGINS:
         for cw in (False, True):
-            print(f"[4-byte tiles] nside={nside} rows={rows} origin={o} cw={cw}", flush=True)
+            print(f"[split tiles] nside={nside} rows={rows} {dtype} origin={o} cw={cw}", flush=True)
             for kind in ("nest2flat", "flat2nest"):
                 got = _call(kind, xd, nside, o, cw).cpu().numpy()
                 np.testing.assert_array_equal(got, _expected(kind, x.numpy(), nside, o, cw),
@@ -182,6 +184,27 @@ def test_nest_flat_4byte_ragged_rows(nside, rows):
     got = cuhpx_amd.nest2flat(xf, "N", True, nside)
     exp = _expected("nest2flat", xf.view(torch.int32).cpu().numpy(), nside, "N", True)
     np.testing.assert_array_equal(got.view(torch.int32).cpu().numpy(), exp)
+
+
+@pytest.mark.parametrize("nside,rows,dtype", [(128, 9, torch.int32), (256, 13, torch.int16), (128, 11, torch.uint8),
+                                              (128, 9, torch.int64), (512, 3, torch.int64)])
+def test_ring_kinds_split_tiles_ragged_rows(nside, rows, dtype):
+    """The RING kinds from nside 128 run 64 x 64 tiles whose loads and stores
+    are done by separate halves of the work-group, eight rows per work-group
+    (except NEST -> RING at nside >= 512 with pixels of at most 4 bytes, and
+    flat -> RING with 4-byte pixels there): row counts that leave a short
+    last chunk, every element size, all four RING kinds, two flat origins."""
+    x = _payload(nside, rows).to(dtype)
+    xd = x.cuda()
+    print(f"[ring split tiles] nside={nside} rows={rows} {dtype}", flush=True)
+    for kind in ("ring2nest", "nest2ring"):
+        got = _call(kind, xd, nside).cpu().numpy()
+        np.testing.assert_array_equal(got, _expected(kind, x.numpy(), nside), err_msg=f"{kind} {nside} {rows}")
+    for o, cw in (("N", True), ("E", False)):
+        for kind in ("ring2flat", "flat2ring"):
+            got = _call(kind, xd, nside, o, cw).cpu().numpy()
+            np.testing.assert_array_equal(got, _expected(kind, x.numpy(), nside, o, cw),
+                                          err_msg=f"{kind} {nside} {rows} {o}{cw}")
 
 
 @pytest.mark.parametrize("nside", [512, 1024])
