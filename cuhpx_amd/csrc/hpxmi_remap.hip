@@ -16,9 +16,13 @@
 // source and destination index of its tile's pixels once, in registers, and
 // then streams the batch rows through LDS: the source side is read in its
 // own natural order (coalesced), the destination side is written in its own
-// natural order (coalesced).  NEST -> RING at nside >= 512 uses tiles that
-// follow the rings instead (nest2ring_bands), so that its RING writes come
-// in runs of 1-2 KB.
+// natural order (coalesced).  From nside 64 (RING kinds: 128) the tiles are
+// 64 x 64 and each work-group's waves split by role (remap_tiles_split): half
+// of them load row r into one LDS buffer while the other half stores row
+// r - 1 from the other.  NEST -> RING at nside >= 512 uses tiles that follow
+// the rings instead (nest2ring_bands), so that its RING writes come in runs
+// of 1-2 KB; flat -> RING with 4-byte pixels there keeps 128 x 128 tiles
+// with both roles in every wave (remap_tiles).
 
 #include "hpxmi_common.h"
 
