@@ -19,10 +19,9 @@
 // natural order (coalesced).  From nside 64 (RING kinds: 128) the tiles are
 // 64 x 64 and each work-group's waves split by role (remap_tiles_split): half
 // of them load row r into one LDS buffer while the other half stores row
-// r - 1 from the other.  NEST -> RING at nside >= 512 uses tiles that follow
-// the rings instead (nest2ring_bands), so that its RING writes come in runs
-// of 1-2 KB; flat -> RING with 4-byte pixels there keeps 128 x 128 tiles
-// with both roles in every wave (remap_tiles).
+// r - 1 from the other.  NEST -> RING at nside >= 512, and flat -> RING with
+// 4-byte pixels there, use tiles that follow the rings instead
+// (nest2ring_bands), so that their RING writes come in runs of 1-2 KB.
 
 #include "hpxmi_common.h"
 
@@ -169,12 +168,12 @@ __device__ __forceinline__ void element(const Side& side, const TileGeom& g, int
   }
 }
 
-// TBMAX: largest tile side (64 or 128; the work-group has TBMAX^2 / PT
-// threads, PT pixels each).  Ring segments in a tile average TB/2 pixels, so
-// the RING kinds use 128-pixel tiles (two 64 KB LDS buffers for 4-byte
-// pixels) to write or read longer runs.  NTL / NTS: non-temporal loads /
-// stores, for a side that streams whole lines of the tile (NEST or flat); the
-// partial-line runs of a RING side go through the L2, which merges them.
+// Maps below nside 64 (RING kinds: 128), where the tile is the whole face
+// side: TBMAX is the largest tile side (the work-group has TBMAX^2 / PT
+// threads, PT pixels each, both loading and storing).  NTL / NTS:
+// non-temporal loads / stores, for a side that streams whole lines of the
+// tile (NEST or flat); the partial-line runs of a RING side go through the
+// L2, which merges them.
 template <typename T, int TBMAX, int PT, bool NTL, bool NTS>
 __global__ __launch_bounds__(TBMAX * TBMAX / PT) void remap_tiles(
     const T* __restrict__ in, T* __restrict__ out, int64_t nside, int order,
@@ -417,15 +416,26 @@ __device__ __forceinline__ uint32_t band_slot(const uint32_t* q, int i) {
   return (q[i / 2] >> (16 * (i & 1))) & 0xffff;
 }
 
-template <typename T, int BW, int BS>
+// FLAT: the source is a flat layout (orientation flip, k) instead of NEST.
+// Its pixel (ix, iy) lies at row y', column x' of the face's block, with
+// (x', y') = correct(ix, iy); depending on the orientation a row y' is one
+// row iy or one column ix of the face, and either way the band holds one run
+// of at most BS consecutive pixels of it (rings s in [s0, s0 + BS)).  Each
+// such run is read by BS consecutive lanes (slot e: run e / BS, pixel e % BS
+// of it); rows meet the band for iy in [s0 - x0 - BW + 1, s0 + BS - 1 - x0]
+// (BW + BS - 1 of them), columns for ix in [x0, x0 + BW).
+template <typename T, int BW, int BS, bool FLAT = false>
 __global__ __launch_bounds__(kBandThreads) void nest2ring_bands(
     const T* __restrict__ in, T* __restrict__ out, int64_t nside, int xblocks, int sblocks,
-    int64_t nrows, int rows_per_chunk) {
+    int64_t nrows, int rows_per_chunk, int flip = 0, int rot = 0) {
   constexpr int kPitch = BW + 8;
   constexpr int kBlocksPerCol = BS / 8 + 1;
   constexpr int kRingPer = BS * BW / kBandThreads;                   // RING pixels per thread
-  constexpr int kNestPer = (BW / 8) * kBlocksPerCol * 64 / kBandThreads;  // NEST slots per thread
-  static_assert(kRingPer % 2 == 0 && kNestPer % 2 == 0 && BS * kPitch < 0xffff, "band shape");
+  constexpr int kRuns = BW + BS - 1;                                 // flat runs that meet the band
+  constexpr int kFlatPer = ((kRuns * BS + kBandThreads - 1) / kBandThreads + 1) / 2 * 2;
+  constexpr int kNestPer = FLAT ? kFlatPer : (BW / 8) * kBlocksPerCol * 64 / kBandThreads;  // source slots per thread
+  static_assert(kRingPer % 2 == 0 && kNestPer % 2 == 0 && BS * kPitch < 0xffff && 64 % BS == 0,
+                "band shape");
   __shared__ T lds[2][BS * kPitch];
   const int n = (int)nside;
   const int tile = xcd_local_index(blockIdx.x, gridDim.x);
@@ -444,6 +454,35 @@ __global__ __launch_bounds__(kBandThreads) void nest2ring_bands(
   uint32_t src_q[kNestPer / 2] = {}, dst_q[kRingPer / 2] = {};
   int src_i[kNestPer];
   const int face_base = face * n * n;  // < 2^31 for nside <= 8192
+  if constexpr (FLAT) {
+    const int mask = n - 1;
+    // does a row y' of the flat block follow iy (one row of the face)?
+    int ax = 0, ay = 0, bx1 = 1, by1 = 0;
+    correct(ax, ay, flip, rot, mask);
+    correct(bx1, by1, flip, rot, mask);
+    const bool rows_follow_iy = ay == by1;
+#pragma unroll
+    for (int i = 0; i < kNestPer; ++i) {
+      const int e = threadIdx.x + i * kBandThreads;
+      const int run = e / BS;
+      const int t = e - run * BS;  // ring s0 + t of the band
+      int ix, iy;
+      bool ok;
+      if (rows_follow_iy) {
+        iy = s0 - x0 - BW + 1 + run;
+        ix = s0 + t - iy;
+        ok = run < kRuns && iy >= 0 && iy < n && ix >= x0 && ix < x0 + BW;
+      } else {
+        ix = x0 + run;
+        iy = s0 + t - ix;
+        ok = run < BW && iy >= 0 && iy < n;
+      }
+      src_q[i / 2] |= (uint32_t)(ok ? t * kPitch + (ix - x0) : 0xffff) << (16 * (i & 1));
+      int fx = ix, fy = iy;
+      if (ok) correct(fx, fy, flip, rot, mask);
+      src_i[i] = ok ? face_base + fy * n + fx : 0;
+    }
+  } else
 #pragma unroll
   for (int i = 0; i < kNestPer; ++i) {
     const int e = threadIdx.x + i * kBandThreads;
@@ -642,8 +681,9 @@ __global__ __launch_bounds__(kBandThreads) void nest2ring_bands_narrow(
   }
 }
 
-template <typename T, int BW, int BS>
-int launch_bands(const void* in, void* out, int64_t nside, int64_t nrows, hipStream_t stream) {
+template <typename T, int BW, int BS, bool FLAT = false>
+int launch_bands(const void* in, void* out, int64_t nside, int64_t nrows, hipStream_t stream, int flip = 0,
+                 int rot = 0) {
   if (nside < BW) HPX_FAIL(HPX_ERR_UNSUPPORTED, "band tiles need nside >= %d", BW);
   const int xblocks = (int)(nside / BW);
   const int sblocks = (int)((BW + nside) / BS);
@@ -658,13 +698,14 @@ int launch_bands(const void* in, void* out, int64_t nside, int64_t nrows, hipStr
   const int rows_per_chunk = (int)((nrows + chunks - 1) / chunks);
   chunks = (nrows + rows_per_chunk - 1) / rows_per_chunk;
   dim3 grid(ntiles, (unsigned)chunks);
+  static_assert(!(FLAT && sizeof(T) < 4), "flat-source bands are built for 4-byte pixels");
   if constexpr (sizeof(T) < 4)
     nest2ring_bands_narrow<T, BW, BS><<<grid, kBandThreads, 0, stream>>>(
         static_cast<const T*>(in), static_cast<T*>(out), nside, xblocks, sblocks, nrows, rows_per_chunk);
   else
-    nest2ring_bands<T, BW, BS><<<grid, kBandThreads, 0, stream>>>(
+    nest2ring_bands<T, BW, BS, FLAT><<<grid, kBandThreads, 0, stream>>>(
         static_cast<const T*>(in), static_cast<T*>(out), nside, xblocks, sblocks, nrows,
-        rows_per_chunk);
+        rows_per_chunk, flip, rot);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
@@ -739,12 +780,14 @@ int launch(const void* in, void* out, int64_t nside, int64_t nrows, Side src, Si
     }
   }
   if constexpr (sizeof(T) == 4) {
-    if (ring && dst.scheme == kRing && nside >= 512) {
-      // flat -> RING with 4-byte pixels from nside 512: 128 x 128 tiles, whose
-      // RING runs average 64 pixels, with both roles in every wave and at
-      // least 16 rows per work-group (flat2ring at nside 1024, [64, npix]
-      // float32: 1.95 ms, against 2.20 ms with the split 64 x 64 tiles below)
-      return launch_tiles<T, 128, 16, false, false>(in, out, nside, nrows, src, dst, stream, 768, 0, 16);
+    // flat -> RING with 4-byte pixels from nside 512: the ring-following
+    // bands of NEST -> RING with a flat source (flat2ring at nside 1024,
+    // [64, npix] float32: 1.45-1.52 ms for the four origins, against 1.95 ms
+    // with 128 x 128 tiles whose RING runs average 64 pixels, and 2.20 ms
+    // with the split 64 x 64 tiles below; nside 2048: 0.95 -> 0.73 ms)
+    if (src.scheme == kFlat && dst.scheme == kRing) {
+      if (nside >= 1024) return launch_bands<T, 512, 32, true>(in, out, nside, nrows, stream, src.flip, src.k);
+      if (nside == 512) return launch_bands<T, 256, 64, true>(in, out, nside, nrows, stream, src.flip, src.k);
     }
   }
   // Every other kind from nside 64 (RING kinds from 128): 64 x 64 tiles whose
