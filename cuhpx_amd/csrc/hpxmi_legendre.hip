@@ -1154,10 +1154,36 @@ __global__ __launch_bounds__(kThreads) void table_transpose_kernel(const float* 
 // (m, l)) and the user layout a[b][l][m] (complex, m fastest).  A tile is one
 // l, 32 consecutive m and 64 fields: 32 rows of 512 bytes on the aT side and
 // 64 runs of 256 bytes on the user side, transposed through LDS.
+// Tiles run m-tile fastest on one XCD (xcd_local_index), so the user-side
+// lines that two neighbouring tiles share meet in that XCD's L2; degrees run
+// in the order 0, lmax-1, 1, lmax-2, ... so that every XCD's contiguous share
+// holds the same number of tiles below the zero triangle.  (Measured at
+// nside 256, L 513, 64 fields against one l per work-group in grid order:
+// fwd+inv step 637 -> 626 us; non-temporal loads or stores on either side
+// made it slower.)
 // ---------------------------------------------------------------------------
 
 constexpr int kTrM = 32;        // m per tile
 constexpr int kTrCols = 128;    // aT columns per tile (64 fields, re / im)
+
+struct TrTile {
+  int l, m0;
+  int64_t c0;
+};
+
+// The tile of this work-group (false: beyond the last tile).
+__device__ __forceinline__ bool tr_tile(int lmax, int mmax, int64_t ncol, TrTile& t) {
+  const int ntm = (mmax + kTrM - 1) / kTrM;
+  const int64_t nct = (ncol + kTrCols - 1) / kTrCols;
+  const int64_t item = xcd_local_index(blockIdx.x, gridDim.x);
+  if (item >= (int64_t)lmax * ntm * nct) return false;
+  t.m0 = (int)(item % ntm) * kTrM;
+  const int64_t r = item / ntm;
+  t.c0 = (r % nct) * kTrCols;
+  const int lp = (int)(r / nct);
+  t.l = (lp & 1) ? lmax - 1 - (lp >> 1) : (lp >> 1);
+  return true;
+}
 
 __device__ __forceinline__ int64_t at_row(const int64_t* p_off, const int* nl, int kpad, int m,
                                           int l) {
@@ -1173,9 +1199,10 @@ __global__ __launch_bounds__(kThreads) void coeff_to_user_kernel(
     const float* __restrict__ aT, float* __restrict__ alm, const int64_t* __restrict__ p_off,
     const int* __restrict__ nl, int kpad, int lmax, int mmax, int64_t ncol, int64_t nbatch, int lnorm) {
   __shared__ float t[kTrM][kTrCols + 1];
-  const int l = blockIdx.x;
-  const int m0 = blockIdx.y * kTrM;
-  const int64_t c0 = (int64_t)blockIdx.z * kTrCols;
+  TrTile tile;
+  if (!tr_tile(lmax, mmax, ncol, tile)) return;
+  const int l = tile.l, m0 = tile.m0;
+  const int64_t c0 = tile.c0;
   const int mend = min(mmax, l + 1);
   const int tid = threadIdx.x;
   {
@@ -1188,7 +1215,7 @@ __global__ __launch_bounds__(kThreads) void coeff_to_user_kernel(
       for (int j = 0; j < 4; ++j) {
         const int c = cq + 32 * j;
         if (c0 + c < ncol) {
-          const float4 v = *reinterpret_cast<const float4*>(src + c);
+          const floatx4 v = *reinterpret_cast<const floatx4*>(src + c);
           t[r][c] = v.x;
           t[r][c + 1] = v.y;
           t[r][c + 2] = v.z;
@@ -1218,9 +1245,10 @@ __global__ __launch_bounds__(kThreads) void user_to_coeff_kernel(
     const float* __restrict__ alm, float* __restrict__ aT, const int64_t* __restrict__ p_off,
     const int* __restrict__ nl, int kpad, int lmax, int mmax, int64_t ncol, int64_t nbatch) {
   __shared__ float t[kTrM][kTrCols + 1];
-  const int l = blockIdx.x;
-  const int m0 = blockIdx.y * kTrM;
-  const int64_t c0 = (int64_t)blockIdx.z * kTrCols;
+  TrTile tile;
+  if (!tr_tile(lmax, mmax, ncol, tile)) return;
+  const int l = tile.l, m0 = tile.m0;
+  const int64_t c0 = tile.c0;
   const int mend = min(mmax, l + 1);
   if (m0 >= mend) return;
   const int tid = threadIdx.x;
@@ -1231,9 +1259,9 @@ __global__ __launch_bounds__(kThreads) void user_to_coeff_kernel(
     for (int j = 0; j < 8; ++j) {
       const int bs = (tid >> 5) + 8 * j;
       const int64_t b = c0 / 2 + bs;
-      float2 v = make_float2(0.f, 0.f);
+      floatx2 v = {0.f, 0.f};
       if (m < mend && b < nbatch)
-        v = *reinterpret_cast<const float2*>(alm + 2 * ((b * lmax + l) * (int64_t)mmax + m));
+        v = *reinterpret_cast<const floatx2*>(alm + 2 * ((b * lmax + l) * (int64_t)mmax + m));
       t[mm][2 * bs] = v.x;
       t[mm][2 * bs + 1] = v.y;
     }
@@ -1289,9 +1317,10 @@ int launch_vector_table64(const ShtPlan& p, double* D, const double* d_ct, hipSt
   return HPX_OK;
 }
 
-static dim3 transpose_grid(const ShtPlan& p, int64_t ncol) {
-  return dim3((unsigned)p.lmax, (unsigned)((p.mmax + kTrM - 1) / kTrM),
-              (unsigned)((ncol + kTrCols - 1) / kTrCols));
+// One work-group per tile (tr_tile); 0 if the tiles do not fit one grid.
+static unsigned transpose_grid(const ShtPlan& p, int64_t ncol) {
+  const int64_t n = (int64_t)p.lmax * ((p.mmax + kTrM - 1) / kTrM) * ((ncol + kTrCols - 1) / kTrCols);
+  return n > 0x7ffffff0 ? 0u : xcd_grid(n);
 }
 
 int launch_gemm_fwd(const ShtPlan& p, int precision, int ct, const float* X, float* aT,
@@ -1393,8 +1422,9 @@ int launch_coeff_to_user(const ShtPlan& p, const float* aT, float* alm, int64_t 
                          hipStream_t s) {
   if (nbatch <= 0) return HPX_OK;
   const int64_t ncol = col_pad(nbatch);
-  if ((ncol + kTrCols - 1) / kTrCols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  coeff_to_user_kernel<<<transpose_grid(p, ncol), kThreads, 0, s>>>(aT, alm, p.d_p_off, p.d_nl, p.kpad,
+  const unsigned grid = transpose_grid(p, ncol);
+  if (grid == 0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  coeff_to_user_kernel<<<grid, kThreads, 0, s>>>(aT, alm, p.d_p_off, p.d_nl, p.kpad,
                                                                    p.lmax, p.mmax, ncol, nbatch, p.vector);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
@@ -1404,8 +1434,9 @@ int launch_user_to_coeff(const ShtPlan& p, const float* alm, float* aT, int64_t 
                          hipStream_t s) {
   if (nbatch <= 0) return HPX_OK;
   const int64_t ncol = col_pad(nbatch);
-  if ((ncol + kTrCols - 1) / kTrCols > 65535) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  user_to_coeff_kernel<<<transpose_grid(p, ncol), kThreads, 0, s>>>(alm, aT, p.d_p_off, p.d_nl, p.kpad,
+  const unsigned grid = transpose_grid(p, ncol);
+  if (grid == 0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
+  user_to_coeff_kernel<<<grid, kThreads, 0, s>>>(alm, aT, p.d_p_off, p.d_nl, p.kpad,
                                                                    p.lmax, p.mmax, ncol, nbatch);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
