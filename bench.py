@@ -302,12 +302,35 @@ def bench_remaps(args):
         gbs = 8.0 * rows * npix / (ms * 1e-3) / 1e9
         out[name] = {"ms": ms, "GB_s": gbs, "frac_of_peak": gbs / PEAK["hbm_GBs"]}
     worst = min(v["GB_s"] for v in out.values())
-    print(json.dumps({"metric": "remap HBM GB/s, nside=1024, [64, npix] float32 (min over kinds)",
-                      "value": worst, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
-                      "warmup": args.warmup, "higher_is_better": True, "dtype": "f32",
-                      "data": "synthetic N(0,1)", "kinds": out,
-                      "roofline": {"bound": "hbm", "achieved": worst, "peak": PEAK["hbm_GBs"],
-                                   "unit": "GB/s", "frac": worst / PEAK["hbm_GBs"], "traffic": None}}))
+    result = {"metric": "remap HBM GB/s, nside=1024, [64, npix] float32 (min over kinds)",
+              "value": worst, "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+              "warmup": args.warmup, "higher_is_better": True, "dtype": "f32",
+              "data": "synthetic N(0,1)", "kinds": out,
+              "roofline": {"bound": "hbm", "achieved": worst, "peak": PEAK["hbm_GBs"],
+                           "unit": "GB/s", "frac": worst / PEAK["hbm_GBs"], "traffic": None}}
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = remap_cpu_baseline(x[:2].cpu().numpy(), nside)
+    print(json.dumps(result))
+
+
+def remap_cpu_baseline(rows, nside):
+    """The reference has no CPU remap; the vectorised NumPy restatement of its
+    index kernels (the oracle) stands in: per call, the destination index of
+    every pixel and the scatter, one host thread, on a 2-row sample."""
+    from oracle import remap as R
+    kinds = {}
+    for name, sig_args in [("nest2ring", ("nest2ring",)), ("ring2nest", ("ring2nest",)),
+                           ("nest2flat", ("nest2flat", "N", True)),
+                           ("flat2flat", ("flat2flat", "S", False, "W", True))]:
+        t0 = time.perf_counter()
+        sig = R.sigma(sig_args[0], nside, *sig_args[1:])
+        R.apply(rows, sig)
+        dt = time.perf_counter() - t0
+        kinds[name] = {"s": dt, "GB_s": 8.0 * rows.shape[0] * rows.shape[1] / dt / 1e9}
+    return {"value": min(v["GB_s"] for v in kinds.values()), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{rows.shape[0]} rows of [64, 12*{nside}^2] float32 per kind (index computation + "
+                      "scatter per call), NumPy restatement of the reference's index kernels, one host thread",
+            "kinds": kinds}
 
 
 def bench_regrid(args):
