@@ -134,6 +134,30 @@ def test_batch_equals_single_maps():
     torch.testing.assert_close(f[1, 2], isht(a[1, 2]), rtol=0, atol=1e-6)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_batches_over_several_coefficient_column_tiles(precision):
+    """130 fields span three 64-field column tiles of the coefficient
+    transposes and GEMMs (the last one partial); every field's coefficients
+    and maps are bit-identical to a transform of its own 64-, 64- or 2-field
+    slice, and the triangle m > l of every field is zero."""
+    nside, L, M = 16, 33, 29
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=M, precision=precision)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=M, precision=precision)
+    g = torch.Generator(device="cuda").manual_seed(21)
+    x = torch.randn(130, 12 * nside * nside, device="cuda", generator=g)
+    a = sht(x)
+    f = isht(a)
+    for lo, hi in [(0, 64), (64, 128), (128, 130)]:
+        print(f"[column tiles {precision}] fields {lo}:{hi}", flush=True)
+        a_part = sht(x[lo:hi])
+        assert torch.equal(a[lo:hi], a_part)
+        assert torch.equal(f[lo:hi], isht(a_part))
+    l = torch.arange(L)[:, None]
+    m = torch.arange(M)[None, :]
+    assert torch.all(a.cpu()[:, l < m] == 0)
+    assert torch.isfinite(torch.view_as_real(a)).all() and torch.isfinite(f).all()
+
+
 def test_upper_triangle_is_zero_and_input_untouched():
     nside = 4
     sht = cuhpx_amd.SHTCUDA(nside)  # default lmax = 15, mmax = 9
