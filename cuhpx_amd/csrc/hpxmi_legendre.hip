@@ -1424,8 +1424,8 @@ int launch_coeff_to_user(const ShtPlan& p, const float* aT, float* alm, int64_t 
   const int64_t ncol = col_pad(nbatch);
   const unsigned grid = transpose_grid(p, ncol);
   if (grid == 0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  coeff_to_user_kernel<<<grid, kThreads, 0, s>>>(aT, alm, p.d_p_off, p.d_nl, p.kpad,
-                                                                   p.lmax, p.mmax, ncol, nbatch, p.vector);
+  coeff_to_user_kernel<<<grid, kThreads, 0, s>>>(aT, alm, p.d_p_off, p.d_nl, p.kpad, p.lmax, p.mmax, ncol,
+                                                 nbatch, p.vector);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
@@ -1436,8 +1436,8 @@ int launch_user_to_coeff(const ShtPlan& p, const float* alm, float* aT, int64_t 
   const int64_t ncol = col_pad(nbatch);
   const unsigned grid = transpose_grid(p, ncol);
   if (grid == 0) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  user_to_coeff_kernel<<<grid, kThreads, 0, s>>>(alm, aT, p.d_p_off, p.d_nl, p.kpad,
-                                                                   p.lmax, p.mmax, ncol, nbatch);
+  user_to_coeff_kernel<<<grid, kThreads, 0, s>>>(alm, aT, p.d_p_off, p.d_nl, p.kpad, p.lmax, p.mmax, ncol,
+                                                 nbatch);
   HPX_LAUNCH_CHECK();
   return HPX_OK;
 }
