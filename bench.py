@@ -125,6 +125,27 @@ def stage_roofline(name, ms, precision):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_nside256_b64.json")
 
 
+def stage_kernels(shapes, precision):
+    """Kernel names (as rocprofv3 prints them) that each stage launches with the
+    tuned launch shapes, so that a stage time can be matched to the kernel-time
+    summaries: FFT size classes 0 (up to 1024 points) and 1 (2048) at nside 256."""
+    p = {"fp32": 0, "bf16x3": 1, "bf16": 2}[precision]
+
+    def ring(kind, c):
+        w = shapes[f"fft_{kind}_waves"][c]
+        t = shapes[f"fft_{kind}_threads_per_field"][c]
+        lo, hi = [(1, 10), (11, 11)][c]
+        return f"ring_{kind}_wave_kernel<{w}, {lo}, {hi}, {t}>"
+
+    ct = shapes["gemm_fwd_col_tiles"]
+    fwd = f"legendre_fwd_w8_kernel<{p}>" if ct == 4 else f"legendre_fwd_kernel<{p}, {ct}, false>"
+    inv = f"legendre_inv_kernel<{p}, {shapes['gemm_inv_col_tiles']}, {shapes['gemm_inv_row_blocks']}, false>"
+    return {"ring_analysis": [ring("analysis", 0), ring("analysis", 1)],
+            "legendre_analysis": [fwd, "coeff_to_user_kernel"],
+            "legendre_synthesis": ["user_to_coeff_kernel", inv],
+            "ring_synthesis": [ring("synthesis", 0), ring("synthesis", 1)]}
+
+
 def measured_traffic(stage):
     """HBM bytes per launch of a stage's kernels from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), or None."""
@@ -572,6 +593,7 @@ def main():
     dom = names[int(np.argmax(stage_ms))]
     roof = dict(stage_roofs[dom])
     roof["traffic"] = measured_traffic(dom)
+    roof["kernels"] = stage_kernels(tuned[args.precision], args.precision)[dom]
 
     result = {
         "metric": "fwd+inv SHT maps/s, nside=256 batch 64: split-BF16 MFMA vs FP32 @ rel-L2<=2e-5",
