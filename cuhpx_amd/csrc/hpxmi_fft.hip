@@ -31,6 +31,8 @@
 // synthesis writes it) directly.
 // Rings that need an 8192- or 16384-point FFT (nside > 512) use a
 // work-group-wide variant of the same Stockham passes (up to 128 KB of LDS).
+// Even rings whose pair would need a 32768-point Bluestein FFT (nside 4096)
+// are transformed one ring at a time as half as many complex points.
 
 #include "hpxmi_sht_internal.h"
 #include "../../include/hpxmi.h"
@@ -730,6 +732,123 @@ __device__ void block_ring_dft(float2* buf, const RingPair& pr, const float2* __
   }
 }
 
+// Rings transformed one at a time as H = N/2 complex points z_j = f_2j +
+// i f_2j+1 (RingPair::half: the pair's packed sequence would need a longer
+// Bluestein FFT than LDS holds).  With Z = DFT_H(z), the real ring's DFT is
+//   F[m] = E[m] + e^{-2 pi i m / N} O[m],  E = (Z[m] + conj Z[H-m]) / 2,
+//   O = (Z[m] - conj Z[H-m]) / (2i)   (indices mod H; m <= H),
+// and its inverse packs Y[m] = E'[m] + i O'[m] with E' = F[m] + F[m+H],
+// O' = (F[m] - F[m+H]) e^{2 pi i m / N}, F[m+H] = conj F[H-m], so that
+// f_2n + i f_2n+1 = IDFT_H(Y)[n] = conj(DFT_H(conj Y))[n].
+__device__ void analysis_half(float2* buf, const StageArgs& a, const RingPair& pr, int k, int64_t b,
+                              const float* __restrict__ fr, float* __restrict__ X) {
+  const int N = pr.nphi, M = pr.M, H = N >> 1;
+  const float2* chirp = a.chirp + pr.chirp_off;
+  const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
+  const int K = min(a.mmax, N / 2 + 1);
+  const float2* ph = a.phase + (int64_t)k * a.mmax;
+  const int64_t pstride = (int64_t)a.kpad * a.ncol;
+  for (int side = 0; side < (pr.single ? 1 : 2); ++side) {
+    const float* src = fr + (side ? pr.off_s : pr.off_n);
+    for (int j = threadIdx.x; j < M; j += kBlockThreads) {
+      float2 z = make_float2(0.f, 0.f);
+      if (j < H) z = cmul(make_float2(src[2 * j], src[2 * j + 1]), chirp[j]);
+      buf[j] = z;
+    }
+    __syncthreads();
+    block_ring_dft(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab);
+    for (int m = threadIdx.x; m < a.mmax; m += kBlockThreads) {
+      float2 F = make_float2(0.f, 0.f);
+      if (m < K) {
+        const float2 g1 = dft_value<false>(buf, pr, m == H ? 0 : m, chirp);
+        const float2 g2 = dft_value<false>(buf, pr, m == 0 ? 0 : H - m, chirp);
+        const float2 E = make_float2(0.5f * (g1.x + g2.x), 0.5f * (g1.y - g2.y));
+        const float2 O = make_float2(0.5f * (g1.y + g2.y), -0.5f * (g1.x - g2.x));
+        float sn, cs;
+        sincospif(-2.0f * (float)m / (float)N, &sn, &cs);
+        F = cadd(E, cmul(make_float2(cs, sn), O));
+        const float c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0f * wscale : wscale;
+        F = cscale(cmul(F, ph[m]), c);
+      }
+      if (a.layout == 0) {
+        // the north ring's spectrum is parked in S and D, then folded with
+        // the south ring's by the same thread
+        float2* S = reinterpret_cast<float2*>(X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * b);
+        float2* D = reinterpret_cast<float2*>(reinterpret_cast<float*>(S) + pstride);
+        if (side == 0) {
+          *S = F;
+          *D = F;
+        } else {
+          const float2 Fn = *S;
+          *S = cadd(Fn, F);
+          *D = csub(Fn, F);
+        }
+      } else {
+        float2* out = reinterpret_cast<float2*>(X) + b * a.nring * (int64_t)a.mmax;
+        out[(int64_t)(side ? a.nring - 1 - k : k) * a.mmax + m] = F;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ void synthesis_half(float2* buf, const StageArgs& a, const RingPair& pr, int k, int64_t b,
+                               const float* __restrict__ X, float* __restrict__ fr) {
+  const int N = pr.nphi, M = pr.M, H = N >> 1;
+  const float2* chirp = a.chirp + pr.chirp_off;
+  const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
+  const int K = min(a.mmax, N / 2 + 1);
+  const float2* ph = a.phase + (int64_t)k * a.mmax;
+  const int64_t pstride = (int64_t)a.kpad * a.ncol;
+  for (int side = 0; side < (pr.single ? 1 : 2); ++side) {
+    // F[m] of this ring for 0 <= m <= H (zero from K on), as the pair
+    // packing forms it: weight and phase applied, halved off DC and Nyquist
+    // in cmode 1, DC and Nyquist real
+    auto spec = [&](int m) -> float2 {
+      if (m >= K) return make_float2(0.f, 0.f);
+      float2 G;
+      if (a.layout == 0) {
+        const float* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * b;
+        const float2 S = *reinterpret_cast<const float2*>(base);
+        const float2 D = *reinterpret_cast<const float2*>(base + pstride);
+        G = (side == 0 || pr.single) ? cadd(S, D) : csub(S, D);
+      } else {
+        const float2* in = reinterpret_cast<const float2*>(X) + b * a.nring * (int64_t)a.mmax;
+        G = in[(int64_t)(side ? a.nring - 1 - k : k) * a.mmax + m];
+      }
+      const bool edge = (m == 0) || (2 * m == N);
+      float c = wscale;
+      if (a.cmode == 1 && !edge) c *= 0.5f;
+      float2 x = cmul(cscale(G, c), cconj(ph[m]));
+      if (edge) x.y = 0.f;
+      return x;
+    };
+    for (int m = threadIdx.x; m < M; m += kBlockThreads) {
+      float2 y = make_float2(0.f, 0.f);
+      if (m < H) {
+        const float2 f1 = spec(m);
+        const float2 f2 = cconj(spec(H - m));  // F[m + H]
+        const float2 Ep = cadd(f1, f2);
+        float sn, cs;
+        sincospif(2.0f * (float)m / (float)N, &sn, &cs);
+        const float2 Op = cmul(csub(f1, f2), make_float2(cs, sn));
+        const float2 Y = make_float2(Ep.x - Op.y, Ep.y + Op.x);
+        y = cmul(cconj(Y), chirp[m]);
+      }
+      buf[m] = y;
+    }
+    __syncthreads();
+    block_ring_dft(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab);
+    float* dst = fr + (side ? pr.off_s : pr.off_n);
+    for (int j = threadIdx.x; j < H; j += kBlockThreads) {
+      const float2 r = dft_value<false>(buf, pr, j, chirp);
+      dst[2 * j] = r.x;
+      dst[2 * j + 1] = -r.y;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(kBlockThreads) void ring_analysis_block_kernel(
     const float* __restrict__ f, float* __restrict__ X, StageArgs a) {
   extern __shared__ float2 buf[];  // the class's longest FFT
@@ -741,6 +860,10 @@ __global__ __launch_bounds__(kBlockThreads) void ring_analysis_block_kernel(
   const bool blue = M != N;
   const float2* chirp = a.chirp + pr.chirp_off;
   const float* fr = f + b * a.npix;
+  if (pr.half) {
+    analysis_half(buf, a, pr, k, b, fr, X);
+    return;
+  }
   for (int j = threadIdx.x; j < M; j += kBlockThreads) {
     float2 z = make_float2(0.f, 0.f);
     if (j < N) {
@@ -789,6 +912,10 @@ __global__ __launch_bounds__(kBlockThreads) void ring_synthesis_block_kernel(
   const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
   const float2* ph = a.phase + (int64_t)k * a.mmax;
+  if (pr.half) {
+    synthesis_half(buf, a, pr, k, b, X, f + b * a.npix);
+    return;
+  }
   for (int idx = threadIdx.x; idx < M; idx += kBlockThreads) buf[idx] = make_float2(0.f, 0.f);
   __syncthreads();
   const int64_t pstride = (int64_t)a.kpad * a.ncol;

@@ -158,6 +158,14 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     const int N = nphi[k];
     r.nphi = N;
     r.M = is_pow2(N) ? N : (int)(int64_t(1) << ilog2(2 * (int64_t)N - 1));
+    // An even ring whose pair transform would need a longer Bluestein FFT
+    // than LDS holds is transformed ring by ring as N/2 complex points
+    // (z_j = f_2j + i f_2j+1), which halves the Bluestein length.
+    r.half = 0;
+    if (r.M > kFftLds && N % 2 == 0 && (int64_t(1) << ilog2((int64_t)N - 1)) <= kFftLds) {
+      r.half = 1;
+      r.M = (int)(int64_t(1) << ilog2((int64_t)N - 1));
+    }
     if (r.M > kFftLds) {
       free_plan(p);
       HPX_FAIL(HPX_ERR_UNSUPPORTED, "ring of %d pixels needs an FFT of %d > %d points", N, r.M, kFftLds);
@@ -174,16 +182,17 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     if (r.M != N) {
       r.chirp_off = (int)chirp.size();
       r.bhat_off = (int)bhat.size();
-      std::vector<cd> w(N);
-      for (int j = 0; j < N; ++j) {
-        const int64_t q = ((int64_t)j * j) % (2 * (int64_t)N);
-        const double ang = -M_PI * (double)q / (double)N;
+      const int Nc = r.half ? N / 2 : N;  // length of the transformed sequence
+      std::vector<cd> w(Nc);
+      for (int j = 0; j < Nc; ++j) {
+        const int64_t q = ((int64_t)j * j) % (2 * (int64_t)Nc);
+        const double ang = -M_PI * (double)q / (double)Nc;
         w[j] = cd(std::cos(ang), std::sin(ang));
         chirp.push_back(to_f2(w[j]));
       }
       std::vector<cd> b(r.M, cd(0.0, 0.0));
       b[0] = std::conj(w[0]);
-      for (int j = 1; j < N; ++j) {
+      for (int j = 1; j < Nc; ++j) {
         b[j] = std::conj(w[j]);
         b[r.M - j] = std::conj(w[j]);
       }
@@ -431,7 +440,7 @@ int healpix_plan(int nside, int lmax, int mmax, const double* ring_weights, hipS
                  hpx_sht_plan** plan, int vector, int csphase, int vnorm = 0, int vinverse = 0) {
   if (plan == nullptr || ring_weights == nullptr) HPX_FAIL(HPX_ERR_ARG, "null argument");
   if (nside < 1) HPX_FAIL(HPX_ERR_ARG, "nside must be >= 1");
-  if (nside > 2048) HPX_FAIL(HPX_ERR_UNSUPPORTED, "nside > 2048 exceeds the FFT size limit");
+  if (nside > 4096) HPX_FAIL(HPX_ERR_UNSUPPORTED, "nside > 4096 exceeds the FFT size limit");
   const int nring = 4 * nside - 1;
   std::vector<int32_t> nphi(nring);
   std::vector<double> z(nring), phi0(nring);

@@ -42,7 +42,8 @@ constexpr int kGemmRhChoices[2] = {1, 2};
 struct RingPair {
   int nphi;       // pixels on each ring of the pair (equal by symmetry)
   int M;          // FFT length: nphi when nphi is a power of two, else the
-                  // Bluestein length pow2 >= 2*nphi - 1
+                  // Bluestein length pow2 >= 2*nphi - 1 (pow2 >= nphi - 1 for
+                  // half pairs)
   int log2M;
   int single;     // 1 for the self-mirrored equator ring
   int64_t off_n;  // first pixel of the north ring
@@ -52,6 +53,9 @@ struct RingPair {
   float weight;   // ring quadrature weight
   int mwrite;     // the analysis writes the spectra for m < mwrite (what the
                   // forward GEMM reads of this pair; zero from min(mmax, nphi/2+1))
+  int half;       // 1: each ring on its own as nphi/2 complex points
+                  // z_j = f_2j + i f_2j+1 (Bluestein of length M >= nphi - 1;
+                  // chirp and kernel spectrum for nphi/2 points)
 };
 
 struct ShtPlan {

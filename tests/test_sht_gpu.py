@@ -303,10 +303,95 @@ def test_nside1024_default_band_limit():
 
 
 def test_unsupported_nside_is_a_clean_error():
-    """nside 4096 needs 32768-point FFTs, more than LDS holds: plan creation
-    refuses it."""
+    """nside 8192 needs 32768-point FFTs even ring by ring (its equator rings
+    hold 32768 pixels), more than LDS holds: plan creation refuses it."""
     with pytest.raises(NotImplementedError):
-        ShtPlan.healpix(4096, 9, 9)
+        ShtPlan.healpix(8192, 9, 9)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_nside4096_ring_by_ring_ffts(precision):
+    """nside 4096 (201 million pixels): the polar rings of 8196..16380 pixels
+    would need 32768-point Bluestein FFTs as mirror pairs and are transformed
+    ring by ring as half-length complex sequences.  lmax = mmax = 17, one
+    field, against the streamed FP64 oracle."""
+    nside, L = 4096, 17
+    print(f"[nside4096 {precision}] oracle", flush=True)
+    o = S.SHTOracle.healpix(nside, L, L, streamed=True)
+    alm = band_limited_alm(L, L, 1, 10)
+    f64 = o.inverse(alm)
+    f32 = f64.astype(np.float32)
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=L, precision=precision)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=L, precision=precision)
+    print(f"[nside4096 {precision}] GPU", flush=True)
+    a = sht(torch.from_numpy(f32).cuda()).cpu().numpy()
+    back = isht(torch.from_numpy(alm.astype(np.complex64)).cuda()).cpu().numpy()
+    a_ref, g_ref = o.forward_and_inverse(f32.astype(np.float64), alm.astype(np.complex64).astype(np.complex128))
+    err_f, err_i = rel_l2(a, a_ref), rel_l2(back, g_ref)
+    print(f"[nside4096 {precision}] forward rel-L2 {err_f:.3e}  inverse rel-L2 {err_i:.3e}", flush=True)
+    assert err_f < TOL[precision] and err_i < TOL[precision]
+
+
+@pytest.mark.parametrize("nphi", [8194, 10002])
+def test_ring_by_ring_ffts_on_a_ring_grid(nphi):
+    """A five-ring grid of nphi-pixel rings (even, > 8192: every pair, the
+    self-mirrored equator included, runs ring by ring as nphi/2 complex
+    points): forward, inverse and both adjoints (exact and compat) against the
+    FP64 oracle."""
+    from oracle.geometry import RingGrid
+    lmax = mmax = 12
+    theta = np.array([0.3, 0.9, np.pi / 2, np.pi - 0.9, np.pi - 0.3])
+    nph = np.full(5, nphi)
+    phi0 = np.full(5, 0.37)
+    w = np.array([0.7, 1.1, 1.3, 1.1, 0.7]) * 4 * np.pi / (nphi * 4.9)
+    grid = RingGrid(theta, nph, phi0, w)
+    o = S.SHTOracle(grid, lmax, mmax)
+    plan = ShtPlan.rings(nph, theta, phi0, w, lmax, mmax)
+    rng = np.random.default_rng(15)
+    f = rng.standard_normal((2, grid.npix))
+    a = rng.standard_normal((2, lmax, mmax)) + 1j * rng.standard_normal((2, lmax, mmax))
+    a[:, np.arange(lmax)[:, None] < np.arange(mmax)[None, :]] = 0
+    f32, a64 = f.astype(np.float32), a.astype(np.complex64)
+    ft, at = torch.from_numpy(f32).cuda(), torch.from_numpy(a64).cuda()
+    f32d, a64d = f32.astype(np.float64), a64.astype(np.complex128)
+    checks = [("forward", plan.forward(ft, "fp32"), o.forward(f32d)),
+              ("inverse", plan.inverse(at, "fp32"), o.inverse(a64d)),
+              ("forward adjoint", plan.forward_adjoint(at, "fp32", "exact"), o.forward_adjoint(a64d)),
+              ("inverse adjoint", plan.inverse_adjoint(ft, "fp32", "exact"), o.inverse_adjoint(f32d)),
+              ("forward adjoint compat", plan.forward_adjoint(at, "fp32", "compat"), o.forward_adjoint_compat(a64d)),
+              ("inverse adjoint compat", plan.inverse_adjoint(ft, "fp32", "compat"), o.inverse_adjoint_compat(f32d))]
+    for name, got, ref in checks:
+        err = rel_l2(got.cpu().numpy(), ref)
+        print(f"[ring grid nphi={nphi}] {name} rel-L2 {err:.2e}", flush=True)
+        assert err < 1e-5, name
+
+
+@pytest.mark.timeout(900)
+def test_nside4096_ring_ffts_match_oracle():
+    """The ring FFT entry points at nside 4096, where the rings of 8196..16380
+    pixels run ring by ring as half-length complex sequences (and the equator
+    rings as one 16384-point pair transform), against the oracle's per-ring
+    FFT: forward spectra and the synthesis of random spectra, every ring."""
+    from cuhpx_amd import cuhpx_fft
+    from oracle.geometry import RingGrid
+    nside, L = 4096, 40
+    grid = RingGrid.healpix(nside, "none")
+    rng = np.random.default_rng(14)
+    f = rng.standard_normal((1, grid.npix)).astype(np.float32)
+    print("[nside4096 rfft] GPU", flush=True)
+    ftm = cuhpx_fft.healpix_rfft_batch(torch.from_numpy(f).cuda(), L, nside).cpu().numpy()
+    print("[nside4096 rfft] oracle", flush=True)
+    ref = S.ring_rfft(f.astype(np.float64), grid, L)
+    err = np.abs(ftm - ref).max() / np.abs(ref).max()
+    print(f"[nside4096 rfft] max error / max |F| {err:.2e}", flush=True)
+    assert err < 3e-6
+    spec = (rng.standard_normal((1, grid.nring, L)) + 1j * rng.standard_normal((1, grid.nring, L))).astype(np.complex64)
+    back = cuhpx_fft.healpix_irfft_batch(torch.from_numpy(spec).cuda(), L, nside).cpu().numpy()
+    ref = S.ring_irfft(spec.astype(np.complex128), grid, L)
+    err = np.abs(back - ref).max() / np.abs(ref).max()
+    print(f"[nside4096 irfft] max error / max |f| {err:.2e}", flush=True)
+    assert err < 3e-6
 
 
 @pytest.mark.timeout(900)
