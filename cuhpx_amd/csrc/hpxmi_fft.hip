@@ -31,8 +31,9 @@
 // synthesis writes it) directly.
 // Rings that need an 8192- or 16384-point FFT (nside > 512) use a
 // work-group-wide variant of the same Stockham passes (up to 128 KB of LDS).
-// Even rings whose pair would need a 32768-point Bluestein FFT (nside 4096)
-// are transformed one ring at a time as half as many complex points.
+// Even rings whose pair would need a longer FFT than LDS holds (nside 4096
+// and 8192) are transformed one ring at a time as a half- or quarter-length
+// complex sequence.
 
 #include "hpxmi_sht_internal.h"
 #include "../../include/hpxmi.h"
@@ -725,24 +726,58 @@ __device__ void block_fft(float2* buf, int log2M, const float2* __restrict__ tw,
 __device__ void block_ring_dft(float2* buf, const RingPair& pr, const float2* __restrict__ bhat,
                                const float2* __restrict__ tw, int log2_tab) {
   block_fft(buf, pr.log2M, tw, log2_tab);
-  if (pr.M != pr.nphi) {
+  if (pr.M != pr.nseq) {
     for (int x = threadIdx.x; x < pr.M; x += kBlockThreads) buf[x] = cconj(cmul(buf[x], bhat[x]));
     __syncthreads();
     block_fft(buf, pr.log2M, tw, log2_tab);
   }
 }
 
-// Rings transformed one at a time as H = N/2 complex points z_j = f_2j +
-// i f_2j+1 (RingPair::half: the pair's packed sequence would need a longer
-// Bluestein FFT than LDS holds).  With Z = DFT_H(z), the real ring's DFT is
+// Rings transformed one at a time (RingPair::half: the pair's packed
+// sequence would need a longer FFT than LDS holds).  The ring's N samples
+// become H = N/2 complex points z_j = f_2j + i f_2j+1.  With Z = DFT_H(z), the
+// real ring's DFT is
 //   F[m] = E[m] + e^{-2 pi i m / N} O[m],  E = (Z[m] + conj Z[H-m]) / 2,
 //   O = (Z[m] - conj Z[H-m]) / (2i)   (indices mod H; m <= H),
 // and its inverse packs Y[m] = E'[m] + i O'[m] with E' = F[m] + F[m+H],
 // O' = (F[m] - F[m+H]) e^{2 pi i m / N}, F[m+H] = conj F[H-m], so that
 // f_2n + i f_2n+1 = IDFT_H(Y)[n] = conj(DFT_H(conj Y))[n].
+// half = 1: DFT_H in one transform (Bluestein or a power of two).
+// half = 2: DFT_H by one radix-2 step over two Bluestein DFTs of Q = H/2
+// points, A of the even and B of the odd points:
+//   DFT_H(u)[x] = A[x mod Q] + e^{-2 pi i x / H} B[x mod Q];
+// the first is kept in registers (Q <= kFftLds / 2 points over the
+// work-group) while the second runs in LDS.
+constexpr int kHalfRegs = kFftLds / 2 / kBlockThreads;
+
+// DFT value x of the sequence the last block_ring_dft transformed.
+__device__ __forceinline__ float2 seq_value(const float2* buf, const RingPair& pr, int x,
+                                            const float2* __restrict__ chirp) {
+  const float2 v = buf[x];
+  return pr.M == pr.nseq ? v : cmul(chirp[x], cconj(v));
+}
+
+// In LDS: sequence j < n of a ring transform, from load(j) (n <= M; zero
+// padded to M), times the chirp for Bluestein; then the transform.
+template <class Load>
+__device__ __forceinline__ void ring_seq_dft(float2* buf, const StageArgs& a, const RingPair& pr,
+                                             const float2* __restrict__ chirp, Load load) {
+  const bool blue = pr.M != pr.nseq;
+  for (int j = threadIdx.x; j < pr.M; j += kBlockThreads) {
+    float2 z = make_float2(0.f, 0.f);
+    if (j < pr.nseq) {
+      z = load(j);
+      if (blue) z = cmul(z, chirp[j]);
+    }
+    buf[j] = z;
+  }
+  __syncthreads();
+  block_ring_dft(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab);
+}
+
 __device__ void analysis_half(float2* buf, const StageArgs& a, const RingPair& pr, int k, int64_t b,
                               const float* __restrict__ fr, float* __restrict__ X) {
-  const int N = pr.nphi, M = pr.M, H = N >> 1;
+  const int N = pr.nphi, H = N >> 1, Q = pr.nseq;
   const float2* chirp = a.chirp + pr.chirp_off;
   const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
@@ -750,18 +785,43 @@ __device__ void analysis_half(float2* buf, const StageArgs& a, const RingPair& p
   const int64_t pstride = (int64_t)a.kpad * a.ncol;
   for (int side = 0; side < (pr.single ? 1 : 2); ++side) {
     const float* src = fr + (side ? pr.off_s : pr.off_n);
-    for (int j = threadIdx.x; j < M; j += kBlockThreads) {
-      float2 z = make_float2(0.f, 0.f);
-      if (j < H) z = cmul(make_float2(src[2 * j], src[2 * j + 1]), chirp[j]);
-      buf[j] = z;
+    if (pr.half == 1) {
+      ring_seq_dft(buf, a, pr, chirp, [&](int j) { return make_float2(src[2 * j], src[2 * j + 1]); });
+    } else {
+      // A = DFT_Q(z_0, z_2, ...), kept in registers; B = DFT_Q(z_1, z_3, ...)
+      // in LDS at x < Q, and A moved to Q + x
+      float2 Areg[kHalfRegs];
+      ring_seq_dft(buf, a, pr, chirp, [&](int j) { return make_float2(src[4 * j], src[4 * j + 1]); });
+#pragma unroll
+      for (int i = 0; i < kHalfRegs; ++i) {
+        const int x = threadIdx.x + i * kBlockThreads;
+        if (x < Q) Areg[i] = seq_value(buf, pr, x, chirp);
+      }
+      __syncthreads();
+      ring_seq_dft(buf, a, pr, chirp, [&](int j) { return make_float2(src[4 * j + 2], src[4 * j + 3]); });
+#pragma unroll
+      for (int i = 0; i < kHalfRegs; ++i) {
+        const int x = threadIdx.x + i * kBlockThreads;
+        if (x < Q) {
+          buf[x] = seq_value(buf, pr, x, chirp);
+          buf[Q + x] = Areg[i];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    block_ring_dft(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab);
+    // Z[x] for 0 <= x < H
+    auto zval = [&](int x) -> float2 {
+      if (pr.half == 1) return seq_value(buf, pr, x, chirp);
+      const int xq = x < Q ? x : x - Q;
+      float sn, cs;
+      sincospif(-2.0f * (float)x / (float)H, &sn, &cs);
+      return cadd(buf[Q + xq], cmul(make_float2(cs, sn), buf[xq]));
+    };
     for (int m = threadIdx.x; m < a.mmax; m += kBlockThreads) {
       float2 F = make_float2(0.f, 0.f);
       if (m < K) {
-        const float2 g1 = dft_value<false>(buf, pr, m == H ? 0 : m, chirp);
-        const float2 g2 = dft_value<false>(buf, pr, m == 0 ? 0 : H - m, chirp);
+        const float2 g1 = zval(m == H ? 0 : m);
+        const float2 g2 = zval(m == 0 ? 0 : H - m);
         const float2 E = make_float2(0.5f * (g1.x + g2.x), 0.5f * (g1.y - g2.y));
         const float2 O = make_float2(0.5f * (g1.y + g2.y), -0.5f * (g1.x - g2.x));
         float sn, cs;
@@ -794,7 +854,7 @@ __device__ void analysis_half(float2* buf, const StageArgs& a, const RingPair& p
 
 __device__ void synthesis_half(float2* buf, const StageArgs& a, const RingPair& pr, int k, int64_t b,
                                const float* __restrict__ X, float* __restrict__ fr) {
-  const int N = pr.nphi, M = pr.M, H = N >> 1;
+  const int N = pr.nphi, H = N >> 1, Q = pr.nseq;
   const float2* chirp = a.chirp + pr.chirp_off;
   const float wscale = (a.use_weight ? pr.weight : 1.0f) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
@@ -823,27 +883,50 @@ __device__ void synthesis_half(float2* buf, const StageArgs& a, const RingPair& 
       if (edge) x.y = 0.f;
       return x;
     };
-    for (int m = threadIdx.x; m < M; m += kBlockThreads) {
-      float2 y = make_float2(0.f, 0.f);
-      if (m < H) {
-        const float2 f1 = spec(m);
-        const float2 f2 = cconj(spec(H - m));  // F[m + H]
-        const float2 Ep = cadd(f1, f2);
-        float sn, cs;
-        sincospif(2.0f * (float)m / (float)N, &sn, &cs);
-        const float2 Op = cmul(csub(f1, f2), make_float2(cs, sn));
-        const float2 Y = make_float2(Ep.x - Op.y, Ep.y + Op.x);
-        y = cmul(cconj(Y), chirp[m]);
-      }
-      buf[m] = y;
-    }
-    __syncthreads();
-    block_ring_dft(buf, pr, a.bhat + pr.bhat_off, a.tw, a.log2_tab);
+    // u[j] = conj Y[j], j < H
+    auto uval = [&](int j) -> float2 {
+      const float2 f1 = spec(j);
+      const float2 f2 = cconj(spec(H - j));  // F[j + H]
+      const float2 Ep = cadd(f1, f2);
+      float sn, cs;
+      sincospif(2.0f * (float)j / (float)N, &sn, &cs);
+      const float2 Op = cmul(csub(f1, f2), make_float2(cs, sn));
+      return cconj(make_float2(Ep.x - Op.y, Ep.y + Op.x));
+    };
     float* dst = fr + (side ? pr.off_s : pr.off_n);
-    for (int j = threadIdx.x; j < H; j += kBlockThreads) {
-      const float2 r = dft_value<false>(buf, pr, j, chirp);
-      dst[2 * j] = r.x;
-      dst[2 * j + 1] = -r.y;
+    if (pr.half == 1) {
+      ring_seq_dft(buf, a, pr, chirp, uval);
+      for (int j = threadIdx.x; j < H; j += kBlockThreads) {
+        const float2 r = seq_value(buf, pr, j, chirp);
+        dst[2 * j] = r.x;
+        dst[2 * j + 1] = -r.y;
+      }
+    } else {
+      // DFT_H(u)[x] = A[x] + w^x B[x] and [x + Q] = A[x] - w^x B[x] (x < Q,
+      // w = e^{-2 pi i / H}), A, B the DFTs of u's even and odd points
+      float2 Areg[kHalfRegs];
+      ring_seq_dft(buf, a, pr, chirp, [&](int j) { return uval(2 * j); });
+#pragma unroll
+      for (int i = 0; i < kHalfRegs; ++i) {
+        const int x = threadIdx.x + i * kBlockThreads;
+        if (x < Q) Areg[i] = seq_value(buf, pr, x, chirp);
+      }
+      __syncthreads();
+      ring_seq_dft(buf, a, pr, chirp, [&](int j) { return uval(2 * j + 1); });
+#pragma unroll
+      for (int i = 0; i < kHalfRegs; ++i) {
+        const int x = threadIdx.x + i * kBlockThreads;
+        if (x < Q) {
+          float sn, cs;
+          sincospif(-2.0f * (float)x / (float)H, &sn, &cs);
+          const float2 wb = cmul(make_float2(cs, sn), seq_value(buf, pr, x, chirp));
+          const float2 r0 = cadd(Areg[i], wb), r1 = csub(Areg[i], wb);
+          dst[2 * x] = r0.x;
+          dst[2 * x + 1] = -r0.y;
+          dst[2 * (x + Q)] = r1.x;
+          dst[2 * (x + Q) + 1] = -r1.y;
+        }
+      }
     }
     __syncthreads();
   }

@@ -157,15 +157,23 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     RingPair& r = p->pairs[k];
     const int N = nphi[k];
     r.nphi = N;
-    r.M = is_pow2(N) ? N : (int)(int64_t(1) << ilog2(2 * (int64_t)N - 1));
-    // An even ring whose pair transform would need a longer Bluestein FFT
-    // than LDS holds is transformed ring by ring as N/2 complex points
-    // (z_j = f_2j + i f_2j+1), which halves the Bluestein length.
+    // FFT length for a sequence of n points: n itself when a power of two,
+    // else the Bluestein length
+    auto fft_len = [](int64_t n) { return is_pow2(n) ? n : int64_t(1) << ilog2(2 * n - 1); };
+    r.M = (int)std::min<int64_t>(fft_len(N), int64_t(1) << 30);
+    // A ring whose pair transform would need a longer FFT than LDS holds is
+    // transformed ring by ring as N/2 complex points z_j = f_2j + i f_2j+1
+    // (half = 1), and if that is still too long, as the two interleaved
+    // halves of z, N/4 points each (half = 2, Bluestein only).
     r.half = 0;
-    if (r.M > kFftLds && N % 2 == 0 && (int64_t(1) << ilog2((int64_t)N - 1)) <= kFftLds) {
+    if (r.M > kFftLds && N % 2 == 0 && fft_len(N / 2) <= kFftLds) {
       r.half = 1;
-      r.M = (int)(int64_t(1) << ilog2((int64_t)N - 1));
+      r.M = (int)fft_len(N / 2);
+    } else if (r.M > kFftLds && N % 4 == 0 && !is_pow2(N / 4) && fft_len(N / 4) <= kFftLds) {
+      r.half = 2;
+      r.M = (int)fft_len(N / 4);
     }
+    r.nseq = N >> r.half;
     if (r.M > kFftLds) {
       free_plan(p);
       HPX_FAIL(HPX_ERR_UNSUPPORTED, "ring of %d pixels needs an FFT of %d > %d points", N, r.M, kFftLds);
@@ -179,10 +187,10 @@ int build_plan(int nring, const int32_t* nphi, const double* cos_theta, const do
     r.chirp_off = 0;
     r.bhat_off = 0;
     max_M = std::max(max_M, r.M);
-    if (r.M != N) {
+    if (r.M != r.nseq) {
       r.chirp_off = (int)chirp.size();
       r.bhat_off = (int)bhat.size();
-      const int Nc = r.half ? N / 2 : N;  // length of the transformed sequence
+      const int Nc = r.nseq;  // length of the transformed sequence
       std::vector<cd> w(Nc);
       for (int j = 0; j < Nc; ++j) {
         const int64_t q = ((int64_t)j * j) % (2 * (int64_t)Nc);
@@ -440,7 +448,7 @@ int healpix_plan(int nside, int lmax, int mmax, const double* ring_weights, hipS
                  hpx_sht_plan** plan, int vector, int csphase, int vnorm = 0, int vinverse = 0) {
   if (plan == nullptr || ring_weights == nullptr) HPX_FAIL(HPX_ERR_ARG, "null argument");
   if (nside < 1) HPX_FAIL(HPX_ERR_ARG, "nside must be >= 1");
-  if (nside > 4096) HPX_FAIL(HPX_ERR_UNSUPPORTED, "nside > 4096 exceeds the FFT size limit");
+  if (nside > 8192) HPX_FAIL(HPX_ERR_UNSUPPORTED, "nside > 8192 exceeds the FFT size limit");
   const int nring = 4 * nside - 1;
   std::vector<int32_t> nphi(nring);
   std::vector<double> z(nring), phi0(nring);

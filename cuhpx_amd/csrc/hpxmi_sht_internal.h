@@ -54,8 +54,10 @@ struct RingPair {
   int mwrite;     // the analysis writes the spectra for m < mwrite (what the
                   // forward GEMM reads of this pair; zero from min(mmax, nphi/2+1))
   int half;       // 1: each ring on its own as nphi/2 complex points
-                  // z_j = f_2j + i f_2j+1 (Bluestein of length M >= nphi - 1;
-                  // chirp and kernel spectrum for nphi/2 points)
+                  // z_j = f_2j + i f_2j+1; 2: z as its even and odd halves,
+                  // two sequences of nphi/4 points (0: the pair packed)
+  int nseq;       // points per transformed sequence (nphi >> half); the
+                  // Bluestein tables are built for it when M != nseq
 };
 
 struct ShtPlan {

@@ -107,7 +107,7 @@ enum {
  * which is generated on the device in FP64 by three-term recurrence with
  * underflow scaling (replacing sht_tools.py:232-264).  Replaces the plan
  * state of HealpixFFT / HealpixIFFT (src/harmonic_transform/hpx_fft.h:69-255).
- * nside up to 4096 (larger: HPX_ERR_UNSUPPORTED, the ring FFTs exceed LDS).
+ * nside up to 8192 (larger: HPX_ERR_UNSUPPORTED).
  * Synchronises the stream before returning. */
 int hpx_sht_plan_create_healpix(int nside, int lmax, int mmax,
                                 const double* ring_weights,

@@ -303,10 +303,43 @@ def test_nside1024_default_band_limit():
 
 
 def test_unsupported_nside_is_a_clean_error():
-    """nside 8192 needs 32768-point FFTs even ring by ring (its equator rings
-    hold 32768 pixels), more than LDS holds: plan creation refuses it."""
-    with pytest.raises(NotImplementedError):
-        ShtPlan.healpix(8192, 9, 9)
+    """nside 16384 is beyond the ring FFTs (and the shipped ring weights):
+    plan creation refuses it, in the library too."""
+    with pytest.raises((NotImplementedError, ValueError)):
+        ShtPlan.healpix(16384, 9, 9)
+    import ctypes
+    lib = _lib.lib()
+    w = np.ones(4 * 16384 - 1)
+    h = ctypes.c_void_p()
+    rc = lib.hpx_sht_plan_create_healpix(16384, 9, 9, w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         _lib.stream_ptr(torch.device("cuda")), ctypes.byref(h))
+    assert rc == _lib.HPX_ERR_UNSUPPORTED and not h.value
+
+
+@pytest.mark.timeout(900)
+def test_nside8192_ring_by_ring_ffts():
+    """nside 8192 (805 million pixels): the equator rings (32768 pixels) run
+    ring by ring as 16384 complex points, the polar rings of 16388..32764
+    pixels as two interleaved 8191..4097-point Bluestein DFTs each, the
+    shorter rings as before.  lmax = mmax = 9, one field, FP32, against the
+    streamed FP64 oracle."""
+    nside, L = 8192, 9
+    print("[nside8192] oracle", flush=True)
+    o = S.SHTOracle.healpix(nside, L, L, streamed=True)
+    alm = band_limited_alm(L, L, 1, 11)
+    f64 = o.inverse(alm)
+    f32 = f64.astype(np.float32)
+    del f64
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=L, precision="fp32")
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=L, precision="fp32")
+    print("[nside8192] GPU", flush=True)
+    a = sht(torch.from_numpy(f32).cuda()).cpu().numpy()
+    back = isht(torch.from_numpy(alm.astype(np.complex64)).cuda()).cpu().numpy()
+    print("[nside8192] oracle check", flush=True)
+    a_ref, g_ref = o.forward_and_inverse(f32.astype(np.float64), alm.astype(np.complex64).astype(np.complex128))
+    err_f, err_i = rel_l2(a, a_ref), rel_l2(back, g_ref)
+    print(f"[nside8192 fp32] forward rel-L2 {err_f:.3e}  inverse rel-L2 {err_i:.3e}", flush=True)
+    assert err_f < TOL["fp32"] and err_i < TOL["fp32"]
 
 
 @pytest.mark.timeout(900)
@@ -333,12 +366,13 @@ def test_nside4096_ring_by_ring_ffts(precision):
     assert err_f < TOL[precision] and err_i < TOL[precision]
 
 
-@pytest.mark.parametrize("nphi", [8194, 10002])
+@pytest.mark.parametrize("nphi", [8194, 10002, 32768, 20004])
 def test_ring_by_ring_ffts_on_a_ring_grid(nphi):
     """A five-ring grid of nphi-pixel rings (even, > 8192: every pair, the
     self-mirrored equator included, runs ring by ring as nphi/2 complex
-    points): forward, inverse and both adjoints (exact and compat) against the
-    FP64 oracle."""
+    points, by Bluestein or, for 32768, one power-of-two FFT; 20004 = 4 x 5001
+    as two interleaved 5001-point Bluestein DFTs per ring): forward, inverse
+    and both adjoints (exact and compat) against the FP64 oracle."""
     from oracle.geometry import RingGrid
     lmax = mmax = 12
     theta = np.array([0.3, 0.9, np.pi / 2, np.pi - 0.9, np.pi - 0.3])
