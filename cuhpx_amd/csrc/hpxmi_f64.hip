@@ -664,6 +664,10 @@ extern "C" int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream) {
   if (p->f64) return HPX_OK;
   if (p->max_M > kMaxM64)
     HPX_FAIL(HPX_ERR_UNSUPPORTED, "FP64 ring FFTs are limited to %d points (this plan needs %d)", kMaxM64, p->max_M);
+  for (int k = 0; k < p->npair; ++k)
+    if (p->pairs[k].half != 0)  // (such rings need more than kMaxM64 points: refused above)
+      HPX_FAIL(HPX_ERR_UNSUPPORTED, "FP64 ring FFTs transform ring pairs whole (ring %d has %d pixels)", k,
+               p->pairs[k].nphi);
   const int mmax = p->mmax;
   // scalar plans: P; vector plans: D0 and D1 (D1 at vstride)
   const int64_t table = std::max<int64_t>((p->vector ? 2 : 1) * p->p_off[mmax], 1);
