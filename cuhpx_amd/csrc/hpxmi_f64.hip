@@ -193,12 +193,12 @@ __device__ void fft64(double2* buf, int log2M, const double2* __restrict__ tw, i
   }
 }
 
-// DFT of length N of the sequence in buf (points x < N; zero beyond): a
+// DFT of length N = pr.nseq of the sequence in buf (points x < N; zero beyond): a
 // power-of-two FFT, or Bluestein: X[x] = chirp_x conj(c_x),
 // c = FFT(conj(FFT(z chirp) bhat)), bhat = FFT(conj chirp) / M.
 __device__ void ring_dft64(double2* buf, const RingPair& pr, const double2* __restrict__ chirp,
                            const double2* __restrict__ bhat, const double2* __restrict__ tw, int log2_tab) {
-  const int N = pr.nphi, M = pr.M;
+  const int N = pr.nseq, M = pr.M;
   if (M == N) {
     fft64(buf, pr.log2M, tw, log2_tab);
     return;
@@ -227,6 +227,165 @@ struct Args64 {
   double scale;
 };
 
+// Rings transformed one at a time (RingPair::half, set for the FP64 FFT
+// length limit): the FP32 kernels' ring-by-ring forms in FP64.  The ring's
+// N samples are H = N/2 complex points z_j = f_2j + i f_2j+1 with
+//   F[m] = E[m] + e^{-2 pi i m / N} O[m],  E = (Z[m] + conj Z[H-m]) / 2,
+//   O = (Z[m] - conj Z[H-m]) / (2i),  Z = DFT_H(z)  (indices mod H),
+// inversely f_2n + i f_2n+1 = conj(DFT_H(conj Y))[n], Y = E' + i O',
+// E' = F[m] + F[m+H], O' = (F[m] - F[m+H]) e^{2 pi i m / N}; half = 2 takes
+// DFT_H as A[x mod Q] + e^{-2 pi i x / H} B[x mod Q] over the Q = H/2-point
+// DFTs of z's even and odd points (A kept in registers).
+constexpr int kHalfRegs64 = kMaxM64 / 2 / kT64;
+
+template <class Load>
+__device__ __forceinline__ void ring_seq_dft64(double2* buf, const Args64& a, const RingPair& pr, Load load) {
+  for (int j = threadIdx.x; j < pr.M; j += kT64) buf[j] = j < pr.nseq ? load(j) : make_double2(0.0, 0.0);
+  __syncthreads();
+  ring_dft64(buf, pr, a.chirp + pr.chirp_off, a.bhat + pr.bhat_off, a.tw, a.log2_tab);
+}
+
+__device__ void analysis_half64(double2* buf, const Args64& a, const RingPair& pr, int k, int64_t b,
+                                const double* __restrict__ fr, double* __restrict__ X) {
+  const int N = pr.nphi, H = N >> 1, Q = pr.nseq;
+  const double wscale = (a.use_weight ? a.w[k] : 1.0) * a.scale;
+  const int K = min(a.mmax, N / 2 + 1);
+  const double2* ph = a.phase + (int64_t)k * a.mmax;
+  const int64_t pstride = (int64_t)a.kpad * a.ncol;
+  for (int side = 0; side < (pr.single ? 1 : 2); ++side) {
+    const double* src = fr + (side ? pr.off_s : pr.off_n);
+    if (pr.half == 1) {
+      ring_seq_dft64(buf, a, pr, [&](int j) { return make_double2(src[2 * j], src[2 * j + 1]); });
+    } else {
+      double2 Areg[kHalfRegs64];
+      ring_seq_dft64(buf, a, pr, [&](int j) { return make_double2(src[4 * j], src[4 * j + 1]); });
+#pragma unroll
+      for (int i = 0; i < kHalfRegs64; ++i) {
+        const int x = threadIdx.x + i * kT64;
+        if (x < Q) Areg[i] = buf[x];
+      }
+      __syncthreads();
+      ring_seq_dft64(buf, a, pr, [&](int j) { return make_double2(src[4 * j + 2], src[4 * j + 3]); });
+#pragma unroll
+      for (int i = 0; i < kHalfRegs64; ++i) {
+        const int x = threadIdx.x + i * kT64;
+        if (x < Q) buf[Q + x] = Areg[i];
+      }
+      __syncthreads();
+    }
+    auto zval = [&](int x) -> double2 {
+      if (pr.half == 1) return buf[x];
+      const int xq = x < Q ? x : x - Q;
+      double sn, cs;
+      sincospi(-2.0 * x / H, &sn, &cs);
+      return zadd(buf[Q + xq], zmul(make_double2(cs, sn), buf[xq]));
+    };
+    for (int m = threadIdx.x; m < a.mmax; m += kT64) {
+      double2 F = make_double2(0.0, 0.0);
+      if (m < K) {
+        const double2 g1 = zval(m == H ? 0 : m);
+        const double2 g2 = zval(m == 0 ? 0 : H - m);
+        const double2 E = make_double2(0.5 * (g1.x + g2.x), 0.5 * (g1.y - g2.y));
+        const double2 O = make_double2(0.5 * (g1.y + g2.y), -0.5 * (g1.x - g2.x));
+        double sn, cs;
+        sincospi(-2.0 * m / N, &sn, &cs);
+        F = zadd(E, zmul(make_double2(cs, sn), O));
+        const double c = (a.cmode == 1 && m != 0 && 2 * m != N) ? 2.0 * wscale : wscale;
+        F = zscale(zmul(F, ph[m]), c);
+      }
+      if (a.layout == 0) {
+        double2* S = reinterpret_cast<double2*>(X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * b);
+        double2* D = reinterpret_cast<double2*>(reinterpret_cast<double*>(S) + pstride);
+        if (side == 0) {
+          *S = F;
+          *D = F;
+        } else {
+          const double2 Fn = *S;
+          *S = zadd(Fn, F);
+          *D = zsub(Fn, F);
+        }
+      } else {
+        double2* out = reinterpret_cast<double2*>(X) + b * a.nring * (int64_t)a.mmax;
+        out[(int64_t)(side ? a.nring - 1 - k : k) * a.mmax + m] = F;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+__device__ void synthesis_half64(double2* buf, const Args64& a, const RingPair& pr, int k, int64_t b,
+                                 const double* __restrict__ X, double* __restrict__ fr) {
+  const int N = pr.nphi, H = N >> 1, Q = pr.nseq;
+  const double wscale = (a.use_weight ? a.w[k] : 1.0) * a.scale;
+  const int K = min(a.mmax, N / 2 + 1);
+  const double2* ph = a.phase + (int64_t)k * a.mmax;
+  const int64_t pstride = (int64_t)a.kpad * a.ncol;
+  for (int side = 0; side < (pr.single ? 1 : 2); ++side) {
+    auto spec = [&](int m) -> double2 {
+      if (m >= K) return make_double2(0.0, 0.0);
+      double2 G;
+      if (a.layout == 0) {
+        const double* base = X + (int64_t)m * 2 * pstride + (int64_t)k * a.ncol + 2 * b;
+        const double2 S = *reinterpret_cast<const double2*>(base);
+        const double2 D = *reinterpret_cast<const double2*>(base + pstride);
+        G = (side == 0 || pr.single) ? zadd(S, D) : zsub(S, D);
+      } else {
+        const double2* in = reinterpret_cast<const double2*>(X) + b * a.nring * (int64_t)a.mmax;
+        G = in[(int64_t)(side ? a.nring - 1 - k : k) * a.mmax + m];
+      }
+      const bool edge = (m == 0) || (2 * m == N);
+      double c = wscale;
+      if (a.cmode == 1 && !edge) c *= 0.5;
+      double2 x = zmul(zscale(G, c), zconj(ph[m]));
+      if (edge) x.y = 0.0;
+      return x;
+    };
+    auto uval = [&](int j) -> double2 {
+      const double2 f1 = spec(j);
+      const double2 f2 = zconj(spec(H - j));
+      const double2 Ep = zadd(f1, f2);
+      double sn, cs;
+      sincospi(2.0 * j / N, &sn, &cs);
+      const double2 Op = zmul(zsub(f1, f2), make_double2(cs, sn));
+      return zconj(make_double2(Ep.x - Op.y, Ep.y + Op.x));
+    };
+    double* dst = fr + (side ? pr.off_s : pr.off_n);
+    if (pr.half == 1) {
+      ring_seq_dft64(buf, a, pr, uval);
+      for (int j = threadIdx.x; j < H; j += kT64) {
+        const double2 r = buf[j];
+        dst[2 * j] = r.x;
+        dst[2 * j + 1] = -r.y;
+      }
+    } else {
+      double2 Areg[kHalfRegs64];
+      ring_seq_dft64(buf, a, pr, [&](int j) { return uval(2 * j); });
+#pragma unroll
+      for (int i = 0; i < kHalfRegs64; ++i) {
+        const int x = threadIdx.x + i * kT64;
+        if (x < Q) Areg[i] = buf[x];
+      }
+      __syncthreads();
+      ring_seq_dft64(buf, a, pr, [&](int j) { return uval(2 * j + 1); });
+#pragma unroll
+      for (int i = 0; i < kHalfRegs64; ++i) {
+        const int x = threadIdx.x + i * kT64;
+        if (x < Q) {
+          double sn, cs;
+          sincospi(-2.0 * x / H, &sn, &cs);
+          const double2 wb = zmul(make_double2(cs, sn), buf[x]);
+          const double2 r0 = zadd(Areg[i], wb), r1 = zsub(Areg[i], wb);
+          dst[2 * x] = r0.x;
+          dst[2 * x + 1] = -r0.y;
+          dst[2 * (x + Q)] = r1.x;
+          dst[2 * (x + Q) + 1] = -r1.y;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // One ring pair x one map per work-group.  layout 0: folded spectra
 // X[m][parity][k][2 b + re/im] (every m < mmax written, zeros from
 // K = min(mmax, N/2+1) on); layout 1: ring-major ftm[b][t][m] (complex).
@@ -240,6 +399,10 @@ __global__ __launch_bounds__(kT64) void ring_analysis64_kernel(const double* __r
   const RingPair pr = a.pairs[k];
   const int N = pr.nphi, M = pr.M;
   const double* fr = f + b * a.npix;
+  if (pr.half) {
+    analysis_half64(buf64, a, pr, k, b, fr, X);
+    return;
+  }
   for (int j = threadIdx.x; j < M; j += kT64) {
     double2 z = make_double2(0.0, 0.0);
     if (j < N) {
@@ -291,6 +454,10 @@ __global__ __launch_bounds__(kT64) void ring_synthesis64_kernel(const double* __
   const double wscale = (a.use_weight ? a.w[k] : 1.0) * a.scale;
   const int K = min(a.mmax, N / 2 + 1);
   const double2* ph = a.phase + (int64_t)k * a.mmax;
+  if (pr.half) {
+    synthesis_half64(buf64, a, pr, k, b, X, f + b * a.npix);
+    return;
+  }
   for (int x = threadIdx.x; x < M; x += kT64) buf64[x] = make_double2(0.0, 0.0);
   __syncthreads();
   const int64_t pstride = (int64_t)a.kpad * a.ncol;
@@ -545,7 +712,7 @@ Args64 make_args64(const ShtPlan& p, int64_t nbatch, int use_weight, double scal
   a.bhat = p.d_bhat64;
   a.tw = p.d_tw64;
   a.w = p.d_w64;
-  a.log2_tab = p.log2_max_M;
+  a.log2_tab = p.log2_max_M64;
   a.npix = p.npix;
   a.nbatch = nbatch;
   a.ncol = col_pad(nbatch);
@@ -564,7 +731,7 @@ int launch_fft64(const ShtPlan& p, bool analysis, const double* in, double* out,
   if (nbatch <= 0) return HPX_OK;
   const int64_t items = (int64_t)p.npair * nbatch;
   if (items > 0x7fffffff) HPX_FAIL(HPX_ERR_UNSUPPORTED, "batch too large");
-  const size_t lds = (size_t)p.max_M * sizeof(double2);
+  const size_t lds = (size_t)p.max_M64 * sizeof(double2);
   static const bool attr = [] {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(ring_analysis64_kernel),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -655,6 +822,11 @@ using namespace hpxmi;
 
 namespace {
 ShtPlan* P64(const hpx_sht_plan* h) { return reinterpret_cast<ShtPlan*>(const_cast<hpx_sht_plan*>(h)); }
+int ceil_log2(int64_t n) {
+  int r = 0;
+  while ((int64_t(1) << r) < n) ++r;
+  return r;
+}
 }  // namespace
 
 extern "C" int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream) {
@@ -662,12 +834,31 @@ extern "C" int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream) {
   if (p == nullptr) HPX_FAIL(HPX_ERR_ARG, "null plan");
   std::lock_guard<std::mutex> g(p->tune_mu);
   if (p->f64) return HPX_OK;
-  if (p->max_M > kMaxM64)
-    HPX_FAIL(HPX_ERR_UNSUPPORTED, "FP64 ring FFTs are limited to %d points (this plan needs %d)", kMaxM64, p->max_M);
-  for (int k = 0; k < p->npair; ++k)
-    if (p->pairs[k].half != 0)  // (such rings need more than kMaxM64 points: refused above)
-      HPX_FAIL(HPX_ERR_UNSUPPORTED, "FP64 ring FFTs transform ring pairs whole (ring %d has %d pixels)", k,
-               p->pairs[k].nphi);
+  // the FP64 kernels' own FFT length per pair (at most kMaxM64 points): the
+  // pair whole, or ring by ring at half or quarter length as in FP32
+  std::vector<RingPair> pairs = p->pairs;
+  int max64 = 1;
+  for (int k = 0; k < p->npair; ++k) {
+    RingPair& r = pairs[k];
+    const int64_t N = r.nphi;
+    auto fft_len = [](int64_t n) { return (n & (n - 1)) == 0 ? n : int64_t(1) << ceil_log2(2 * n - 1); };
+    r.half = 0;
+    int64_t M = fft_len(N);
+    if (M > kMaxM64 && N % 2 == 0 && fft_len(N / 2) <= kMaxM64) {
+      r.half = 1;
+      M = fft_len(N / 2);
+    } else if (M > kMaxM64 && N % 4 == 0 && ((N / 4) & (N / 4 - 1)) != 0 && fft_len(N / 4) <= kMaxM64) {
+      r.half = 2;
+      M = fft_len(N / 4);
+    }
+    if (M > kMaxM64)
+      HPX_FAIL(HPX_ERR_UNSUPPORTED, "FP64 ring FFTs are limited to %d points (a ring of %lld pixels needs %lld)",
+               kMaxM64, (long long)N, (long long)M);
+    r.M = (int)M;
+    r.log2M = ceil_log2(M);
+    r.nseq = (int)(N >> r.half);
+    max64 = std::max(max64, r.M);
+  }
   const int mmax = p->mmax;
   // scalar plans: P; vector plans: D0 and D1 (D1 at vstride)
   const int64_t table = std::max<int64_t>((p->vector ? 2 : 1) * p->p_off[mmax], 1);
@@ -681,11 +872,12 @@ extern "C" int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream) {
   // formulas, kept in FP64): chirp exp(-i pi (j^2 mod 2N) / N), kernel
   // spectrum FFT_M(conj chirp) / M
   std::vector<double2> chirp, bhat;
-  std::vector<RingPair> pairs = p->pairs;
   for (int k = 0; k < p->npair; ++k) {
     RingPair& r = pairs[k];
-    if (r.M == r.nphi) continue;
-    const int N = r.nphi;
+    r.chirp_off = 0;
+    r.bhat_off = 0;
+    if (r.M == r.nseq) continue;
+    const int N = r.nseq;
     r.chirp_off = (int)chirp.size();
     r.bhat_off = (int)bhat.size();
     std::vector<cd> w(N), b(r.M, cd(0.0, 0.0));
@@ -707,9 +899,9 @@ extern "C" int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream) {
     chirp.push_back(make_double2(1.0, 0.0));
     bhat.push_back(make_double2(1.0, 0.0));
   }
-  std::vector<double2> tw(p->max_M), phase((size_t)p->npair * mmax);
-  for (int x = 0; x < p->max_M; ++x) {
-    const double ang = -2.0 * M_PI * (double)x / (double)p->max_M;
+  std::vector<double2> tw(max64), phase((size_t)p->npair * mmax);
+  for (int x = 0; x < max64; ++x) {
+    const double ang = -2.0 * M_PI * (double)x / (double)max64;
     tw[x] = make_double2(std::cos(ang), std::sin(ang));
   }
   for (int k = 0; k < p->npair; ++k)
@@ -775,6 +967,8 @@ extern "C" int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream) {
   p->n_items64 = (int)items.size();
   p->d_p64 = d_p;
   p->d_pairs64 = d_pairs64;
+  p->max_M64 = max64;
+  p->log2_max_M64 = ceil_log2(max64);
   p->table_bytes += table * (int64_t)sizeof(double);
   p->f64 = 1;
   return HPX_OK;

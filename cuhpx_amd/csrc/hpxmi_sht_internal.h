@@ -102,6 +102,8 @@ struct ShtPlan {
   double2* d_phase64 = nullptr;
   int2* d_items64 = nullptr;      // forward FP64 GEMM items (m, parity * 65536 + row tile)
   int n_items64 = 0;
+  int max_M64 = 1;                // largest FP64 FFT length (the FP64 pairs' own M)
+  int log2_max_M64 = 0;
   int f64 = 0;
   // Vector plans hold tables instead:
   float* d_p = nullptr;         // Legendre table [m][parity][row][kpad]

@@ -200,8 +200,9 @@ int hpx_sht_inverse_adjoint(const hpx_sht_plan* plan, int precision, int mode,
  * at nside 256, lmax = mmax = 513, twice that for the two tables of a vector
  * plan; Bluestein tables, twiddles, phases); it allocates, synchronises the
  * stream, and is a no-op on a plan that has it.  The _f64 calls below return
- * HPX_ERR_ARG on a plan without it.  Rings longer than 8192 points
- * (nside > 1024) are refused. */
+ * HPX_ERR_ARG on a plan without it.  FP64 FFTs stop at 8192 points, so
+ * longer rings run ring by ring at half or quarter length (nside 2048 and
+ * 4096); plans that would still need more (nside 8192) are refused. */
 int hpx_sht_plan_enable_f64(hpx_sht_plan* plan, hipStream_t stream);
 
 /* Workspace bytes of an FP64 call on nbatch maps. */

@@ -163,3 +163,58 @@ def test_f64_vector_transforms_match_oracle(nside, lmax, mmax, norm):
     ei = rel_l2(g.cpu().numpy(), o.inverse(a))
     print(f"[f64 vector nside={nside} L={lmax} M={mmax} {norm}] forward {ef:.2e} inverse {ei:.2e}", flush=True)
     assert ef < TOL64 and ei < TOL64
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("nside", [2048, 4096])
+def test_f64_large_nside_ring_by_ring(nside):
+    """FP64 at nside 2048 (polar rings of 4100..8188 pixels ring by ring as
+    half-length sequences: FP64 FFTs stop at 8192 points) and nside 4096
+    (quarter-length Bluestein DFTs, the 16384-pixel equator rings as one
+    8192-point FFT each), lmax = mmax = 9, one field, against the streamed
+    FP64 oracle."""
+    L = 9
+    print(f"[f64 nside={nside}] oracle", flush=True)
+    o = S.SHTOracle.healpix(nside, L, L, streamed=True)
+    alm = band_limited_alm(L, L, 1, 47)
+    f = o.inverse(alm)
+    sht = cuhpx_amd.SHTCUDA(nside, lmax=L, mmax=L)
+    isht = cuhpx_amd.iSHTCUDA(nside, lmax=L, mmax=L)
+    print(f"[f64 nside={nside}] GPU", flush=True)
+    a = sht(torch.from_numpy(f).cuda()).cpu().numpy()
+    g = isht(torch.from_numpy(alm).cuda()).cpu().numpy()
+    print(f"[f64 nside={nside}] oracle check", flush=True)
+    a_ref, g_ref = o.forward_and_inverse(f, alm)
+    ef, ei = rel_l2(a, a_ref), rel_l2(g, g_ref)
+    print(f"[f64 nside={nside}] forward {ef:.2e} inverse {ei:.2e}", flush=True)
+    assert ef < TOL64 and ei < TOL64
+
+
+@pytest.mark.parametrize("nphi", [6002, 12004, 16384])
+def test_f64_ring_by_ring_on_a_ring_grid(nphi):
+    """Five rings of nphi pixels: 6002 runs ring by ring as 3001-point
+    Bluestein DFTs, 12004 as two interleaved 3001-point DFTs per ring, 16384 as
+    one 8192-point FFT per ring (FP64 FFTs stop at 8192 points); forward,
+    inverse and both adjoints against the oracle."""
+    lmax = mmax = 12
+    theta = np.array([0.3, 0.9, np.pi / 2, np.pi - 0.9, np.pi - 0.3])
+    nph = np.full(5, nphi)
+    phi0 = np.full(5, 0.37)
+    w = np.array([0.7, 1.1, 1.3, 1.1, 0.7]) * 4 * np.pi / (nphi * 4.9)
+    grid = RingGrid(theta, nph, phi0, w)
+    o = S.SHTOracle(grid, lmax, mmax)
+    plan = ShtPlan.rings(nph, theta, phi0, w, lmax, mmax)
+    rng = np.random.default_rng(48)
+    f = rng.standard_normal((2, grid.npix))
+    a = rng.standard_normal((2, lmax, mmax)) + 1j * rng.standard_normal((2, lmax, mmax))
+    a[:, np.arange(lmax)[:, None] < np.arange(mmax)[None, :]] = 0
+    ft, at = torch.from_numpy(f).cuda(), torch.from_numpy(a).cuda()
+    checks = [("forward", plan.forward(ft, "fp32"), o.forward(f)),
+              ("inverse", plan.inverse(at, "fp32"), o.inverse(a)),
+              ("forward adjoint", plan.forward_adjoint(at, "fp32", "exact"), o.forward_adjoint(a)),
+              ("inverse adjoint", plan.inverse_adjoint(ft, "fp32", "exact"), o.inverse_adjoint(f))]
+    for name, got, ref in checks:
+        assert got.dtype in (torch.float64, torch.complex128), name
+        err = rel_l2(got.cpu().numpy(), ref)
+        print(f"[f64 ring grid nphi={nphi}] {name} rel-L2 {err:.2e}", flush=True)
+        assert err < TOL64, name
